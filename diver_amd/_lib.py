@@ -1,0 +1,107 @@
+"""ctypes binding of libdiver.so (the C ABI declared in include/diver.h).
+
+The library is loaded from this package directory; if it is missing every call raises
+``DiverUnavailable``: there is no fallback implementation.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, 'libdiver.so')
+
+_lib = None
+
+EXPORTS = [
+    'diver_version', 'diver_last_error', 'diver_ray_voxel_intersect', 'diver_masked_intersect',
+    'diver_aabb_intersect', 'diver_ray_march', 'diver_integrate', 'diver_integrate_backward',
+    'diver_composite', 'diver_composite_backward', 'diver_decoder_create', 'diver_decoder_destroy',
+    'diver_decoder_param_count', 'diver_mlp_eval', 'diver_render_rays', 'diver_render_camera',
+    'diver_camera_rays',
+]
+
+
+class DiverUnavailable(RuntimeError):
+    pass
+
+
+class DiverError(RuntimeError):
+    def __init__(self, fn: str, status: int, msg: str):
+        super().__init__(f'{fn} failed (status {status}): {msg}')
+        self.status = status
+
+
+class DecoderDesc(ctypes.Structure):
+    _fields_ = [('channels', ctypes.c_int), ('hidden', ctypes.c_int), ('n_pre', ctypes.c_int),
+                ('pre_identity', ctypes.c_int), ('density_act', ctypes.c_int),
+                ('n_post', ctypes.c_int)]
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [('voxels', ctypes.c_void_p), ('mask', ctypes.c_void_p), ('n', ctypes.c_int),
+                ('channels', ctypes.c_int), ('xyzmin', ctypes.c_float),
+                ('voxel_size', ctypes.c_float)]
+
+
+def _declare(L):
+    vp, i64, i32, fl = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_float
+    L.diver_version.restype = ctypes.c_char_p
+    L.diver_version.argtypes = []
+    L.diver_last_error.restype = ctypes.c_char_p
+    L.diver_last_error.argtypes = []
+    sig = {
+        'diver_ray_voxel_intersect': [i64, i32, fl, fl, fl, fl, vp, vp, vp, vp, vp, vp],
+        'diver_masked_intersect': [i64, i32, fl, fl, fl, fl, fl, vp, vp, vp, vp, vp, vp, vp],
+        'diver_aabb_intersect': [i64, vp, vp, vp, vp, vp, vp, i32, vp],
+        'diver_ray_march': [i64, vp, vp, vp, vp, i32, i64, vp],
+        'diver_integrate': [vp, i32, i32, i64, vp, vp, vp, vp],
+        'diver_integrate_backward': [i32, i32, i64, vp, vp, vp, vp, vp],
+        'diver_composite': [i64, i32, vp, vp, vp, i32, vp, vp, vp],
+        'diver_composite_backward': [i64, i32, vp, vp, vp, i32, vp, vp, vp, vp, vp],
+        'diver_decoder_create': [ctypes.POINTER(DecoderDesc), ctypes.POINTER(ctypes.c_float), i64,
+                                 ctypes.POINTER(vp)],
+        'diver_decoder_destroy': [vp],
+        'diver_mlp_eval': [vp, i64, vp, vp, vp, i32, vp, vp, vp, fl, fl, vp],
+        'diver_render_rays': [ctypes.POINTER(Grid), vp, i32, i64, vp, vp, fl, vp, vp, vp],
+        'diver_render_camera': [ctypes.POINTER(Grid), vp, i32, ctypes.POINTER(ctypes.c_float), i32,
+                                i32, fl, fl, vp, vp, vp],
+        'diver_camera_rays': [ctypes.POINTER(ctypes.c_float), i32, i32, fl, vp, vp, vp],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+    L.diver_decoder_param_count.argtypes = [ctypes.POINTER(DecoderDesc)]
+    L.diver_decoder_param_count.restype = ctypes.c_int64
+
+
+def lib():
+    """The loaded library (raises DiverUnavailable when it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise DiverUnavailable(
+                f'{LIB_PATH} is missing: build it with `python -m diver_amd.build` '
+                '(the render path has no CPU fallback)')
+        L = ctypes.CDLL(LIB_PATH)
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+def call(name: str, *args) -> None:
+    L = lib()
+    st = getattr(L, name)(*args)
+    if st != 0:
+        raise DiverError(name, st, L.diver_last_error().decode(errors='replace'))
+
+
+def stream_of(t) -> int:
+    """Handle of torch's current stream on the tensor's device."""
+    import torch
+    return torch.cuda.current_stream(t.device).cuda_stream
+
+
+def ptr(t) -> int:
+    return t.data_ptr() if t is not None else None

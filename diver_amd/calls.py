@@ -1,0 +1,218 @@
+"""Tensor-level calls into libdiver (device tensors in, device tensors out).
+
+Each function validates device, dtype and contiguity, then enqueues the library call on torch's
+current stream.  There is no CPU fallback: CPU tensors raise.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _lib
+
+
+def _dev(*ts):
+    dev = None
+    for t in ts:
+        if t is None:
+            continue
+        if not t.is_cuda:
+            raise ValueError('libdiver operates on device tensors; got a CPU tensor')
+        if dev is None:
+            dev = t.device
+        elif t.device != dev:
+            raise ValueError('tensors are on different devices')
+    return dev
+
+
+def _f32(t, name):
+    if t.dtype != torch.float32:
+        raise TypeError(f'{name} must be float32, got {t.dtype}')
+    return t.contiguous()
+
+
+def _u8(t, name):
+    if t.dtype not in (torch.bool, torch.uint8):
+        raise TypeError(f'{name} must be bool/uint8, got {t.dtype}')
+    return t.contiguous()
+
+
+def _i32(t, name):
+    if t.dtype != torch.int32:
+        raise TypeError(f'{name} must be int32, got {t.dtype}')
+    return t.contiguous()
+
+
+def _s(t):
+    return _lib.stream_of(t)
+
+
+def masked_intersect_raw(o, v, mask, xyzmin, voxel_num, voxel_size, mask_scale, max_n,
+                         counts_only=False):
+    """One launch of diver_masked_intersect; returns (isect, num, tns) (isect/tns None when
+    counts_only)."""
+    dev = _dev(o, v, mask)
+    o, v, mask = _f32(o, 'o'), _f32(v, 'v'), _u8(mask, 'mask')
+    B = o.shape[0]
+    num = torch.empty(B, dtype=torch.int32, device=dev)
+    isect = tns = None
+    if not counts_only:
+        isect = torch.empty((B, max_n, 6), dtype=torch.float32, device=dev)
+        tns = torch.empty((B, max_n), dtype=torch.float32, device=dev)
+    _lib.call('diver_masked_intersect', B, int(max_n), float(xyzmin), 0.0, float(voxel_num),
+              float(voxel_size), float(mask_scale), o.data_ptr(), v.data_ptr(), mask.data_ptr(),
+              _lib.ptr(isect), num.data_ptr(), _lib.ptr(tns), _s(o))
+    return isect, num, tns
+
+
+def ray_voxel_intersect_raw(o, v, xyzmin, voxel_num, voxel_size, max_n, counts_only=False):
+    dev = _dev(o, v)
+    o, v = _f32(o, 'o'), _f32(v, 'v')
+    B = o.shape[0]
+    num = torch.empty(B, dtype=torch.int32, device=dev)
+    isect = tns = None
+    if not counts_only:
+        isect = torch.empty((B, max_n, 3), dtype=torch.float32, device=dev)
+        tns = torch.empty((B, max_n), dtype=torch.float32, device=dev)
+    _lib.call('diver_ray_voxel_intersect', B, int(max_n), float(xyzmin), 0.0, float(voxel_num),
+              float(voxel_size), o.data_ptr(), v.data_ptr(), _lib.ptr(isect), num.data_ptr(),
+              _lib.ptr(tns), _s(o))
+    return isect, num, tns
+
+
+def aabb_intersect(coords, v, center, finish, rgba, octrees, voxel_num):
+    _dev(coords, v, center, finish, rgba, octrees)
+    for t, n in ((coords, 'coords'), (v, 'v'), (center, 'center'), (rgba, 'rgba')):
+        if t.dtype != torch.float32 or not t.is_contiguous():
+            raise TypeError(f'{n} must be a contiguous float32 tensor (modified in place)')
+    if finish.dtype not in (torch.bool, torch.uint8) or not finish.is_contiguous():
+        raise TypeError('finish must be a contiguous bool tensor (modified in place)')
+    octrees = _u8(octrees, 'octrees')
+    B = v.numel() // 3
+    _lib.call('diver_aabb_intersect', B, coords.data_ptr(), v.data_ptr(), center.data_ptr(),
+              finish.data_ptr(), rgba.data_ptr(), octrees.data_ptr(), int(voxel_num), _s(v))
+
+
+REF_THREAD_STRIDE = 256 * 1024  # grid-stride width of the reference's ray_march launch
+
+
+def ray_march(coords, v, mask, finish, voxel_num, thread_stride=REF_THREAD_STRIDE):
+    _dev(coords, v, mask, finish)
+    if coords.dtype != torch.float32 or not coords.is_contiguous():
+        raise TypeError('coords must be a contiguous float32 tensor (modified in place)')
+    v, mask, finish = _f32(v, 'v'), _u8(mask, 'mask'), _u8(finish, 'finish')
+    B = v.numel() // 3
+    _lib.call('diver_ray_march', B, coords.data_ptr(), v.data_ptr(), mask.data_ptr(),
+              finish.data_ptr(), int(voxel_num), int(thread_stride), _s(v))
+
+
+def integrate(voxels, coord_in, coord_out):
+    dev = _dev(voxels, coord_in, coord_out)
+    voxels = _f32(voxels, 'voxels')
+    ci, co = _f32(coord_in, 'coord_in'), _f32(coord_out, 'coord_out')
+    n1, C = voxels.shape[0], voxels.shape[-1]
+    K = ci.shape[0]
+    out = torch.empty((K, C), dtype=torch.float32, device=dev)
+    _lib.call('diver_integrate', voxels.data_ptr(), n1, C, K, ci.data_ptr(), co.data_ptr(),
+              out.data_ptr(), _s(voxels))
+    return out
+
+
+def integrate_backward(voxels_shape, coord_in, coord_out, grad_feature, grad_voxels=None):
+    dev = _dev(coord_in, coord_out, grad_feature)
+    ci, co = _f32(coord_in, 'coord_in'), _f32(coord_out, 'coord_out')
+    g = _f32(grad_feature, 'grad_feature')
+    if grad_voxels is None:
+        grad_voxels = torch.zeros(voxels_shape, dtype=torch.float32, device=dev)
+    n1, C = voxels_shape[0], voxels_shape[-1]
+    _lib.call('diver_integrate_backward', n1, C, ci.shape[0], ci.data_ptr(), co.data_ptr(),
+              g.data_ptr(), grad_voxels.data_ptr(), _s(g))
+    return grad_voxels
+
+
+def composite(sigma, color, mask, white_back):
+    dev = _dev(sigma, color, mask)
+    sigma, color, mask = _f32(sigma, 'sigma'), _f32(color, 'color'), _u8(mask, 'mask')
+    B, M = sigma.shape
+    rgb = torch.empty((B, 3), dtype=torch.float32, device=dev)
+    weight = torch.empty((B, M), dtype=torch.float32, device=dev)
+    _lib.call('diver_composite', B, M, sigma.data_ptr(), color.data_ptr(), mask.data_ptr(),
+              int(bool(white_back)), rgb.data_ptr(), weight.data_ptr(), _s(sigma))
+    return rgb, weight
+
+
+def composite_backward(sigma, color, mask, white_back, grad_rgb, grad_weight=None):
+    dev = _dev(sigma, color, mask, grad_rgb)
+    sigma, color, mask = _f32(sigma, 'sigma'), _f32(color, 'color'), _u8(mask, 'mask')
+    grad_rgb = _f32(grad_rgb, 'grad_rgb')
+    if grad_weight is not None:
+        grad_weight = _f32(grad_weight, 'grad_weight')
+    B, M = sigma.shape
+    gs = torch.empty((B, M), dtype=torch.float32, device=dev)
+    gc = torch.empty((B, M, 3), dtype=torch.float32, device=dev)
+    _lib.call('diver_composite_backward', B, M, sigma.data_ptr(), color.data_ptr(),
+              mask.data_ptr(), int(bool(white_back)), grad_rgb.data_ptr(),
+              _lib.ptr(grad_weight), gs.data_ptr(), gc.data_ptr(), _s(sigma))
+    return gs, gc
+
+
+def mlp_eval(decoder, rgba, coord, features, voxel_map, v, finish, alpha_skip, t_stop):
+    _dev(rgba, coord, features, voxel_map, v, finish)
+    if rgba.dtype != torch.float32 or not rgba.is_contiguous():
+        raise TypeError('rgba must be a contiguous float32 tensor (modified in place)')
+    if finish.dtype not in (torch.bool, torch.uint8) or not finish.is_contiguous():
+        raise TypeError('finish must be a contiguous bool tensor (modified in place)')
+    coord, features, v = _f32(coord, 'coord'), _f32(features, 'features'), _f32(v, 'v')
+    voxel_map = _i32(voxel_map, 'voxel_map')
+    B = v.numel() // 3
+    _lib.call('diver_mlp_eval', decoder.handle, B, coord.data_ptr(), features.data_ptr(),
+              voxel_map.data_ptr(), int(voxel_map.shape[0]), v.data_ptr(), rgba.data_ptr(),
+              finish.data_ptr(), float(alpha_skip), float(t_stop), _s(v))
+
+
+PRECISIONS = {'fp32': 0, 'bf16': 1, 'fp16': 2}
+
+
+def grid_struct(voxels, mask, xyzmin, voxel_size):
+    n = mask.shape[0]
+    return _lib.Grid(voxels.data_ptr(), mask.data_ptr(), int(n), int(voxels.shape[-1]),
+                     float(xyzmin), float(voxel_size))
+
+
+def render_rays(voxels, mask, xyzmin, voxel_size, decoder, o, d, precision='fp32', t_stop=1e-4,
+                with_count=True):
+    dev = _dev(voxels, mask, o, d)
+    voxels, mask = _f32(voxels, 'voxels'), _u8(mask, 'mask')
+    o, d = _f32(o, 'o'), _f32(d, 'd')
+    R = o.shape[0]
+    rgba = torch.empty((R, 4), dtype=torch.float32, device=dev)
+    count = torch.empty(R, dtype=torch.int32, device=dev) if with_count else None
+    g = grid_struct(voxels, mask, xyzmin, voxel_size)
+    _lib.call('diver_render_rays', ctypes.byref(g), decoder.handle, PRECISIONS[precision], R,
+              o.data_ptr(), d.data_ptr(), float(t_stop), rgba.data_ptr(), _lib.ptr(count), _s(o))
+    return rgba, count
+
+
+def render_camera(voxels, mask, xyzmin, voxel_size, decoder, c2w, width, height, focal,
+                  precision='fp32', t_stop=1e-4, with_count=True, out=None):
+    dev = _dev(voxels, mask)
+    voxels, mask = _f32(voxels, 'voxels'), _u8(mask, 'mask')
+    rgba = out if out is not None else torch.empty((height, width, 4), dtype=torch.float32,
+                                                   device=dev)
+    count = torch.empty((height, width), dtype=torch.int32, device=dev) if with_count else None
+    g = grid_struct(voxels, mask, xyzmin, voxel_size)
+    m = (ctypes.c_float * 12)(*[float(x) for x in list(c2w.reshape(-1))])
+    _lib.call('diver_render_camera', ctypes.byref(g), decoder.handle, PRECISIONS[precision], m,
+              int(width), int(height), float(focal), float(t_stop), rgba.data_ptr(),
+              _lib.ptr(count), _s(voxels))
+    return rgba, count
+
+
+def camera_rays(c2w, width, height, focal, device):
+    o = torch.empty((height * width, 3), dtype=torch.float32, device=device)
+    d = torch.empty((height * width, 3), dtype=torch.float32, device=device)
+    m = (ctypes.c_float * 12)(*[float(x) for x in list(c2w.reshape(-1))])
+    _lib.call('diver_camera_rays', m, int(width), int(height), float(focal), o.data_ptr(),
+              d.data_ptr(), torch.cuda.current_stream(o.device).cuda_stream)
+    return o, d

@@ -1,0 +1,373 @@
+// C-ABI of libdiver: argument validation, decoder handles, dispatch to the HIP launchers.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/diver.h"
+#include "kernels.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int hip_status(hipError_t e, const char* what) {
+    if (e == hipSuccess) return DIVER_OK;
+    return fail(DIVER_EHIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define REQUIRE(cond, ...)                               \
+    do {                                                 \
+        if (!(cond)) return fail(DIVER_EINVAL, __VA_ARGS__); \
+    } while (0)
+
+hipStream_t S(diver_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+int variant_of(const diver_decoder_desc& d) {
+    if (d.n_pre == 0 && !d.pre_identity && d.density_act == 1 && d.n_post == 1) {
+        if (d.channels == 32 && d.hidden == 32) return 0;
+        if (d.channels == 32 && d.hidden == 64) return 1;
+        if (d.channels == 64 && d.hidden == 64) return 2;
+    }
+    if (d.density_act == 0 && d.n_post == 0 && d.channels == 32) {
+        if (d.hidden == 32 && d.n_pre == 2 && !d.pre_identity) return 3;
+        if (d.hidden == 64 && d.n_pre == 1 && !d.pre_identity) return 4;
+        if (d.hidden == 32 && d.n_pre == 2 && d.pre_identity) return 5;
+    }
+    return -1;
+}
+
+uint16_t to_bf16(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffff) ? 0x40 : 0));
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+uint16_t to_f16(float f) {
+    _Float16 h = (_Float16)f;
+    uint16_t r;
+    std::memcpy(&r, &h, 2);
+    return r;
+}
+
+}  // namespace
+
+struct diver_decoder {
+    diver_decoder_desc desc;
+    int variant;
+    int device;
+    int64_t n_params;
+    float* W = nullptr;            // canonical fp32
+    uint32_t* pack_bf16 = nullptr; // matrix-core A operands
+    uint32_t* pack_f16 = nullptr;
+};
+
+extern "C" {
+
+const char* diver_version(void) { return "libdiver 0.1 gfx950"; }
+
+const char* diver_last_error(void) { return g_err.c_str(); }
+
+int diver_ray_voxel_intersect(int64_t batch, int max_n, float xyzmin, float xyzmax,
+                              float voxel_num, float voxel_size, const float* o, const float* v,
+                              float* isect, int32_t* num, float* tns, diver_stream_t stream) {
+    (void)xyzmax;
+    REQUIRE(batch >= 0, "batch < 0");
+    REQUIRE(batch == 0 || (o && v && num), "null ray or count pointer");
+    REQUIRE(max_n >= 0, "max_n < 0");
+    REQUIRE(voxel_num > 0 && voxel_size > 0, "voxel_num and voxel_size must be positive");
+    return hip_status(diver::launch_ray_voxel_intersect(batch, max_n, xyzmin, voxel_num,
+                                                        voxel_size, o, v, isect, num, tns,
+                                                        S(stream)),
+                      "ray_voxel_intersect");
+}
+
+int diver_masked_intersect(int64_t batch, int max_n, float xyzmin, float xyzmax, float voxel_num,
+                           float voxel_size, float mask_scale, const float* o, const float* v,
+                           const uint8_t* mask, float* isect, int32_t* num, float* tns,
+                           diver_stream_t stream) {
+    (void)xyzmax;
+    REQUIRE(batch >= 0, "batch < 0");
+    REQUIRE(batch == 0 || (o && v && num && mask), "null ray, mask or count pointer");
+    REQUIRE(max_n >= 0, "max_n < 0");
+    REQUIRE(voxel_num > 0 && voxel_size > 0 && mask_scale > 0,
+            "voxel_num, voxel_size and mask_scale must be positive");
+    return hip_status(diver::launch_masked_intersect(batch, max_n, xyzmin, voxel_num, voxel_size,
+                                                     mask_scale, o, v, mask, isect, num, tns,
+                                                     S(stream)),
+                      "masked_intersect");
+}
+
+int diver_aabb_intersect(int64_t batch, float* coords, const float* v, const float* center,
+                         uint8_t* finish, float* rgba, const uint8_t* octrees, int voxel_num,
+                         diver_stream_t stream) {
+    REQUIRE(batch >= 0, "batch < 0");
+    REQUIRE(voxel_num >= 64 && voxel_num % 64 == 0,
+            "voxel_num must be a positive multiple of 64 (got %d)", voxel_num);
+    REQUIRE(batch == 0 || (coords && v && center && finish && rgba && octrees), "null pointer");
+    return hip_status(diver::launch_aabb_intersect(batch, coords, v, center, finish, rgba, octrees,
+                                                   voxel_num, S(stream)),
+                      "aabb_intersect");
+}
+
+int diver_ray_march(int64_t batch, float* coords, const float* v, const uint8_t* mask,
+                    const uint8_t* finish, int voxel_num, int64_t thread_stride,
+                    diver_stream_t stream) {
+    REQUIRE(batch >= 0, "batch < 0");
+    REQUIRE(voxel_num > 0, "voxel_num must be positive");
+    REQUIRE(batch == 0 || (coords && v && mask && finish), "null pointer");
+    if (thread_stride <= 0) thread_stride = batch > 0 ? batch : 1;
+    return hip_status(diver::launch_ray_march(batch, coords, v, mask, finish, voxel_num,
+                                              thread_stride, S(stream)),
+                      "ray_march");
+}
+
+int diver_integrate(const float* voxels, int n1, int channels, int64_t k, const float* coord_in,
+                    const float* coord_out, float* feature, diver_stream_t stream) {
+    REQUIRE(k >= 0 && n1 > 0, "bad sizes");
+    REQUIRE(channels > 0 && channels % 4 == 0, "channels must be a positive multiple of 4");
+    REQUIRE(k == 0 || (voxels && coord_in && coord_out && feature), "null pointer");
+    return hip_status(diver::launch_integrate(voxels, n1, channels, k, coord_in, coord_out,
+                                              feature, S(stream)),
+                      "integrate");
+}
+
+int diver_integrate_backward(int n1, int channels, int64_t k, const float* coord_in,
+                             const float* coord_out, const float* grad_feature,
+                             float* grad_voxels, diver_stream_t stream) {
+    REQUIRE(k >= 0 && n1 > 0, "bad sizes");
+    REQUIRE(channels > 0 && channels % 4 == 0, "channels must be a positive multiple of 4");
+    REQUIRE(k == 0 || (grad_voxels && coord_in && coord_out && grad_feature), "null pointer");
+    return hip_status(diver::launch_integrate_backward(n1, channels, k, coord_in, coord_out,
+                                                       grad_feature, grad_voxels, S(stream)),
+                      "integrate_backward");
+}
+
+int diver_composite(int64_t batch, int m, const float* sigma, const float* color,
+                    const uint8_t* mask, int white_back, float* rgb, float* weight,
+                    diver_stream_t stream) {
+    REQUIRE(batch >= 0 && m >= 0, "bad sizes");
+    REQUIRE(batch == 0 || (rgb && (m == 0 || (sigma && color && mask && weight))), "null pointer");
+    return hip_status(
+        diver::launch_composite(batch, m, sigma, color, mask, white_back, rgb, weight, S(stream)),
+        "composite");
+}
+
+int diver_composite_backward(int64_t batch, int m, const float* sigma, const float* color,
+                             const uint8_t* mask, int white_back, const float* grad_rgb,
+                             const float* grad_weight, float* grad_sigma, float* grad_color,
+                             diver_stream_t stream) {
+    REQUIRE(batch >= 0 && m >= 0, "bad sizes");
+    REQUIRE(batch == 0 || m == 0 || (sigma && color && mask && grad_rgb && grad_sigma && grad_color),
+            "null pointer");
+    return hip_status(diver::launch_composite_backward(batch, m, sigma, color, mask, white_back,
+                                                       grad_rgb, grad_weight, grad_sigma,
+                                                       grad_color, S(stream)),
+                      "composite_backward");
+}
+
+int64_t diver_decoder_param_count(const diver_decoder_desc* d) {
+    if (!d) return -1;
+    const int64_t C = d->channels, H = d->hidden;
+    const int64_t Z = d->n_pre ? H : C;
+    int64_t n = 0;
+    for (int i = 0; i < d->n_pre; ++i) n += (i == 0) ? (d->pre_identity ? H : H * C + H) : H * H + H;
+    n += Z + 1 + H * Z + H * 27 + H;
+    n += (int64_t)d->n_post * (H * H + H);
+    n += 3 * H + 3;
+    return n;
+}
+
+// Matrix-core A operands of one layer W[M][K] (row-major): per (32-row tile, 16-wide k chunk,
+// lane) 8 consecutive k values of row 32*mt + lane%32 starting at 16*kc + 8*(lane/32).
+static void pack_layer(const float* W, int M, int K, bool bf16, std::vector<uint32_t>& out) {
+    const int MT = (M + 31) / 32, KC = K / 16;
+    for (int mt = 0; mt < MT; ++mt)
+        for (int kc = 0; kc < KC; ++kc)
+            for (int lane = 0; lane < 64; ++lane) {
+                int row = 32 * mt + (lane & 31);
+                int k0 = 16 * kc + 8 * (lane >> 5);
+                for (int q = 0; q < 4; ++q) {
+                    float a = row < M ? W[(int64_t)row * K + k0 + 2 * q] : 0.0f;
+                    float b = row < M ? W[(int64_t)row * K + k0 + 2 * q + 1] : 0.0f;
+                    uint16_t ha = bf16 ? to_bf16(a) : to_f16(a);
+                    uint16_t hb = bf16 ? to_bf16(b) : to_f16(b);
+                    out.push_back((uint32_t)ha | ((uint32_t)hb << 16));
+                }
+            }
+}
+
+int diver_decoder_create(const diver_decoder_desc* desc, const float* params_host,
+                         int64_t n_params, diver_decoder** out) {
+    REQUIRE(desc && params_host && out, "null argument");
+    *out = nullptr;
+    int variant = variant_of(*desc);
+    if (variant < 0)
+        return fail(DIVER_EUNSUPPORTED,
+                    "decoder (channels=%d hidden=%d n_pre=%d pre_identity=%d act=%d n_post=%d) is "
+                    "not compiled into this library",
+                    desc->channels, desc->hidden, desc->n_pre, desc->pre_identity,
+                    desc->density_act, desc->n_post);
+    int64_t need = diver_decoder_param_count(desc);
+    REQUIRE(n_params == need, "decoder expects %lld parameters, got %lld", (long long)need,
+            (long long)n_params);
+    for (int64_t i = 0; i < n_params; ++i)
+        REQUIRE(std::isfinite(params_host[i]), "non-finite decoder parameter at %lld", (long long)i);
+
+    diver_decoder* d = new diver_decoder();
+    d->desc = *desc;
+    d->variant = variant;
+    d->n_params = n_params;
+    hipGetDevice(&d->device);
+    auto cleanup = [&](int code, const char* what) {
+        if (d->W) (void)hipFree(d->W);
+        if (d->pack_bf16) (void)hipFree(d->pack_bf16);
+        if (d->pack_f16) (void)hipFree(d->pack_f16);
+        delete d;
+        return fail(code, "%s", what);
+    };
+    if (hipMalloc(&d->W, sizeof(float) * n_params) != hipSuccess)
+        return cleanup(DIVER_ENOMEM, "decoder parameter allocation failed");
+    if (hipMemcpy(d->W, params_host, sizeof(float) * n_params, hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup(DIVER_EHIP, "decoder parameter upload failed");
+
+    if (desc->n_pre == 0) {
+        const int C = desc->channels, H = desc->hidden;
+        const int64_t wc = (int64_t)C + 1;  // w_s[Z], b_s then W_c
+        const int64_t post0 = wc + (int64_t)H * C + (int64_t)H * 27 + H;
+        const int64_t outo = post0 + (int64_t)desc->n_post * (H * H + H);
+        for (int prec = 0; prec < 2; ++prec) {
+            std::vector<uint32_t> pk;
+            pack_layer(params_host + wc, H, C, prec == 0, pk);
+            for (int j = 0; j < desc->n_post; ++j)
+                pack_layer(params_host + post0 + (int64_t)j * (H * H + H), H, H, prec == 0, pk);
+            pack_layer(params_host + outo, 3, H, prec == 0, pk);
+            uint32_t** dst = prec == 0 ? &d->pack_bf16 : &d->pack_f16;
+            if (hipMalloc(dst, pk.size() * 4) != hipSuccess)
+                return cleanup(DIVER_ENOMEM, "decoder pack allocation failed");
+            if (hipMemcpy(*dst, pk.data(), pk.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+                return cleanup(DIVER_EHIP, "decoder pack upload failed");
+        }
+    }
+    *out = d;
+    return DIVER_OK;
+}
+
+int diver_decoder_destroy(diver_decoder* d) {
+    if (!d) return DIVER_OK;
+    int dev = 0;
+    hipGetDevice(&dev);
+    hipSetDevice(d->device);
+    if (d->W) (void)hipFree(d->W);
+    if (d->pack_bf16) (void)hipFree(d->pack_bf16);
+    if (d->pack_f16) (void)hipFree(d->pack_f16);
+    hipSetDevice(dev);
+    delete d;
+    return DIVER_OK;
+}
+
+int diver_mlp_eval(const diver_decoder* dec, int64_t batch, const float* coord,
+                   const float* features, const int32_t* voxel_map, int map_size, const float* v,
+                   float* rgba, uint8_t* finish, float alpha_skip, float t_stop,
+                   diver_stream_t stream) {
+    REQUIRE(dec, "null decoder");
+    REQUIRE(batch >= 0 && map_size > 0, "bad sizes");
+    REQUIRE(batch == 0 || (coord && features && voxel_map && v && rgba && finish), "null pointer");
+    return hip_status(diver::launch_mlp_eval(dec->variant, batch, coord, features, voxel_map,
+                                             map_size, v, rgba, finish, dec->W, alpha_skip, t_stop,
+                                             S(stream)),
+                      "mlp_eval");
+}
+
+static int render_common(const diver_grid* g, const diver_decoder* dec, int precision,
+                         diver::RenderArgs& a) {
+    REQUIRE(g && dec, "null grid or decoder");
+    REQUIRE(g->voxels && g->mask, "null grid arrays");
+    REQUIRE(g->n > 0 && g->voxel_size > 0, "bad grid size");
+    REQUIRE(g->channels == dec->desc.channels, "grid has %d channels, decoder expects %d",
+            g->channels, dec->desc.channels);
+    REQUIRE(precision >= DIVER_FP32 && precision <= DIVER_FP16, "bad precision %d", precision);
+    if (dec->variant == 5)
+        return fail(DIVER_EUNSUPPORTED,
+                    "the real-time blob decoder (pre-multiplied features) is for mlp_eval only");
+    if (precision != DIVER_FP32 && dec->desc.n_pre != 0)
+        return fail(DIVER_EUNSUPPORTED,
+                    "the matrix-core decoder path supports decoders without pre-layers only");
+    a.voxels = g->voxels;
+    a.mask = g->mask;
+    a.n = g->n;
+    a.channels = g->channels;
+    a.xyzmin = g->xyzmin;
+    a.voxel_size = g->voxel_size;
+    a.W = dec->W;
+    a.pack = precision == DIVER_BF16 ? dec->pack_bf16 : dec->pack_f16;
+    return DIVER_OK;
+}
+
+int diver_render_rays(const diver_grid* grid, const diver_decoder* dec, int precision,
+                      int64_t n_rays, const float* o, const float* d, float t_stop, float* rgba,
+                      int32_t* count, diver_stream_t stream) {
+    diver::RenderArgs a{};
+    int st = render_common(grid, dec, precision, a);
+    if (st != DIVER_OK) return st;
+    REQUIRE(n_rays >= 0, "n_rays < 0");
+    REQUIRE(n_rays == 0 || (o && d && rgba), "null pointer");
+    a.n_rays = n_rays;
+    a.o = o;
+    a.d = d;
+    a.cam.width = 0;
+    a.t_stop = t_stop;
+    a.rgba = rgba;
+    a.count = count;
+    return hip_status(diver::launch_render(dec->variant, a, precision, S(stream)), "render_rays");
+}
+
+int diver_render_camera(const diver_grid* grid, const diver_decoder* dec, int precision,
+                        const float* c2w_host, int width, int height, float focal, float t_stop,
+                        float* rgba, int32_t* count, diver_stream_t stream) {
+    diver::RenderArgs a{};
+    int st = render_common(grid, dec, precision, a);
+    if (st != DIVER_OK) return st;
+    REQUIRE(c2w_host && rgba, "null pointer");
+    REQUIRE(width > 0 && height > 0 && focal > 0, "bad camera");
+    std::memcpy(a.cam.c2w, c2w_host, sizeof(a.cam.c2w));
+    a.cam.width = width;
+    a.cam.height = height;
+    a.cam.focal = focal;
+    int64_t tiles = (int64_t)((width + 7) / 8) * ((height + 7) / 8);
+    a.n_rays = tiles * 64;
+    a.t_stop = t_stop;
+    a.rgba = rgba;
+    a.count = count;
+    return hip_status(diver::launch_render(dec->variant, a, precision, S(stream)), "render_camera");
+}
+
+int diver_camera_rays(const float* c2w_host, int width, int height, float focal, float* o,
+                      float* d, diver_stream_t stream) {
+    REQUIRE(c2w_host && o && d, "null pointer");
+    REQUIRE(width > 0 && height > 0 && focal > 0, "bad camera");
+    diver::CameraDesc cam{};
+    std::memcpy(cam.c2w, c2w_host, sizeof(cam.c2w));
+    cam.width = width;
+    cam.height = height;
+    cam.focal = focal;
+    return hip_status(diver::launch_camera_rays(cam, o, d, S(stream)), "camera_rays");
+}
+
+}  // extern "C"

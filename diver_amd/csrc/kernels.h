@@ -1,0 +1,63 @@
+// Internal launcher interface between the C-ABI (capi.cpp) and the HIP kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace diver {
+
+struct CameraDesc {
+    float c2w[12];  // row-major 3x4 camera-to-world
+    int width;      // 0 = explicit rays
+    int height;
+    float focal;
+};
+
+struct RenderArgs {
+    const float* voxels;   // [(N+1)^3][C] fp32
+    const uint8_t* mask;   // [N^3]
+    int n;
+    int channels;
+    float xyzmin;
+    float voxel_size;
+    const float* W;        // canonical decoder params (fp32, device)
+    const uint32_t* pack;  // matrix-core A operands for the chosen precision (device)
+    int64_t n_rays;        // explicit: number of rays; camera: tiles*64
+    const float* o;
+    const float* d;
+    CameraDesc cam;
+    float t_stop;
+    float* rgba;           // [rays][4]: accumulated rgb (no background) and transmittance
+    int32_t* count;        // [rays] intervals consumed (optional)
+};
+
+hipError_t launch_ray_voxel_intersect(int64_t batch, int max_n, float xyzmin, float voxel_num,
+                                      float voxel_size, const float* o, const float* v,
+                                      float* isect, int32_t* num, float* tns, hipStream_t s);
+hipError_t launch_masked_intersect(int64_t batch, int max_n, float xyzmin, float voxel_num,
+                                   float voxel_size, float mask_scale, const float* o,
+                                   const float* v, const uint8_t* mask, float* isect, int32_t* num,
+                                   float* tns, hipStream_t s);
+hipError_t launch_aabb_intersect(int64_t batch, float* coords, const float* v, const float* center,
+                                 uint8_t* finish, float* rgba, const uint8_t* octrees,
+                                 int voxel_num, hipStream_t s);
+hipError_t launch_ray_march(int64_t batch, float* coords, const float* v, const uint8_t* mask,
+                            const uint8_t* finish, int voxel_num, int64_t stride, hipStream_t s);
+hipError_t launch_integrate(const float* voxels, int n1, int C, int64_t K, const float* ci,
+                            const float* co, float* out, hipStream_t s);
+hipError_t launch_integrate_backward(int n1, int C, int64_t K, const float* ci, const float* co,
+                                     const float* gout, float* gvox, hipStream_t s);
+hipError_t launch_composite(int64_t batch, int m, const float* sigma, const float* color,
+                            const uint8_t* mask, int white_back, float* rgb, float* weight,
+                            hipStream_t s);
+hipError_t launch_composite_backward(int64_t batch, int m, const float* sigma, const float* color,
+                                     const uint8_t* mask, int white_back, const float* grad_rgb,
+                                     const float* grad_weight, float* grad_sigma,
+                                     float* grad_color, hipStream_t s);
+hipError_t launch_mlp_eval(int variant, int64_t batch, const float* coord, const float* feats,
+                           const int32_t* vmap, int map_size, const float* v, float* rgba,
+                           uint8_t* finish, const float* W, float alpha_skip, float t_stop,
+                           hipStream_t s);
+hipError_t launch_render(int variant, const RenderArgs& a, int prec, hipStream_t s);
+hipError_t launch_camera_rays(const CameraDesc& cam, float* o, float* d, hipStream_t s);
+
+}  // namespace diver

@@ -1,0 +1,181 @@
+/*
+ * libdiver: MI355X (gfx950) implementation of DIVeR's render-and-train hot path behind a C ABI.
+ *
+ * Conventions for every entry point:
+ *   - pointers are device pointers unless the parameter name ends in _host;
+ *   - work is enqueued on `stream` (a hipStream_t; NULL = the null stream) and the call returns
+ *     without a device-wide synchronisation;
+ *   - the return value is a diver_status (DIVER_OK = 0); diver_last_error() describes the last
+ *     failure of the calling thread; no entry point aborts the process;
+ *   - every output element is written (padding included), so outputs need no pre-fill.
+ * Coordinates are grid units ([0, N] per axis), features are [z][y][x][c] with (N+1)^3 corners,
+ * masks are [z][y][x] bytes (0/1).
+ */
+#ifndef DIVER_H
+#define DIVER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* diver_stream_t;
+
+enum diver_status {
+    DIVER_OK = 0,
+    DIVER_EINVAL = 1,       /* bad argument (null pointer, size, unsupported shape)          */
+    DIVER_ENOMEM = 2,       /* device allocation failed                                       */
+    DIVER_EHIP = 3,         /* HIP runtime error (launch or copy)                             */
+    DIVER_EUNSUPPORTED = 4  /* combination not compiled into the library                      */
+};
+
+const char* diver_version(void);
+const char* diver_last_error(void);
+
+/* ---- traversal (training / offline) ------------------------------------------------------- */
+
+/* Every voxel-boundary crossing of each ray.  Replaces the pybind11 op
+ * ray_voxel_intersect(o, v, xyzmin, xyzmax, voxel_num, voxel_size)
+ * (utils/ray_voxel_intersection/include/ray_voxel_intersect.h:5-9, kernel
+ * src/ray_voxel_intersect.cu:29-151).  isect [batch][max_n][3], tns [batch][max_n]; entries past
+ * the count are 0 / -1.  isect and tns may be NULL to compute counts only.  num[b] is the true
+ * count even when it exceeds max_n (entries past max_n are dropped). */
+int diver_ray_voxel_intersect(int64_t batch, int max_n, float xyzmin, float xyzmax,
+                              float voxel_num, float voxel_size, const float* o, const float* v,
+                              float* isect, int32_t* num, float* tns, diver_stream_t stream);
+
+/* (entry, exit) of every occupied voxel along each ray.  Replaces masked_intersect(o, v, mask,
+ * xyzmin, xyzmax, voxel_num, voxel_size, mask_scale)
+ * (utils/ray_voxel_intersection/include/ray_voxel_intersect.h:11-15, kernel
+ * src/ray_voxel_intersect.cu:192-319).  mask [M^3] with M = int(voxel_num * mask_scale).
+ * isect [batch][max_n][6], tns [batch][max_n] = exit distance; NULL isect/tns = counts only. */
+int diver_masked_intersect(int64_t batch, int max_n, float xyzmin, float xyzmax, float voxel_num,
+                           float voxel_size, float mask_scale, const float* o, const float* v,
+                           const uint8_t* mask, float* isect, int32_t* num, float* tns,
+                           diver_stream_t stream);
+
+/* ---- traversal (real time) ------------------------------------------------------------------ */
+
+/* Box entry + coarse-to-fine octree descent to the first occupied N/4 block; initialises rgba to
+ * (0,0,0,1) and finish.  Replaces aabb_intersect(o, v, center, finish, rgba, octrees, voxel_num)
+ * (utils/ray_march/include/ray_march.h:5-10, kernel src/ray_march.cu:24-149).  coords
+ * [8][batch][6] (slot 7's exit receives the start point, -1 on a miss); center [3] device, grid
+ * units; octrees = max-pooled masks at block sizes 64, 16, 4 concatenated coarse first.
+ * voxel_num must be a positive multiple of 64 (the reference compiles 128 and 256 only). */
+int diver_aabb_intersect(int64_t batch, float* coords, const float* v, const float* center,
+                         uint8_t* finish, float* rgba, const uint8_t* octrees, int voxel_num,
+                         diver_stream_t stream);
+
+/* One march round: up to 8 occupied (entry, exit) pairs per unfinished ray into coords slots
+ * 0..7, -1 terminator.  Replaces ray_march(o, v, mask, finish, voxel_num)
+ * (utils/ray_march/include/ray_march.h:12-16, kernel src/ray_march.cu:151-230).  thread_stride
+ * is the reference's grid-stride width (262144 = 256 x 1024; <= 0 means batch), which decides
+ * which rays a thread skips after meeting a negative resume point. */
+int diver_ray_march(int64_t batch, float* coords, const float* v, const uint8_t* mask,
+                    const uint8_t* finish, int voxel_num, int64_t thread_stride,
+                    diver_stream_t stream);
+
+/* ---- segment integral ----------------------------------------------------------------------- */
+
+/* Mean of the trilinear feature field over each segment.  Replaces integrate(voxels, coord_i,
+ * coord_o) (utils/integrator.py:62-113).  voxels [n1][n1][n1][channels], coords [k][3],
+ * feature [k][channels]; channels must be a multiple of 4. */
+int diver_integrate(const float* voxels, int n1, int channels, int64_t k, const float* coord_in,
+                    const float* coord_out, float* feature, diver_stream_t stream);
+
+/* Adjoint of diver_integrate: grad_voxels += d feature / d voxels ^T grad_feature (atomic
+ * scatter-add; the reference gets this from autograd through integrator.py's gathers). */
+int diver_integrate_backward(int n1, int channels, int64_t k, const float* coord_in,
+                             const float* coord_out, const float* grad_feature,
+                             float* grad_voxels, diver_stream_t stream);
+
+/* ---- compositing ------------------------------------------------------------------------------ */
+
+/* alpha = 1 - exp(-relu(sigma) * mask); weight = alpha * exclusive cumprod(1 - alpha);
+ * rgb = sum weight * color (+ 1 - sum weight with white_back).  Replaces DIVeR.render
+ * (model/diver.py:183-207).  sigma/mask [batch][m], color [batch][m][3], rgb [batch][3]. */
+int diver_composite(int64_t batch, int m, const float* sigma, const float* color,
+                    const uint8_t* mask, int white_back, float* rgb, float* weight,
+                    diver_stream_t stream);
+
+/* Adjoint of diver_composite (grad_weight may be NULL). */
+int diver_composite_backward(int64_t batch, int m, const float* sigma, const float* color,
+                             const uint8_t* mask, int white_back, const float* grad_rgb,
+                             const float* grad_weight, float* grad_sigma, float* grad_color,
+                             diver_stream_t stream);
+
+/* ---- decoder ---------------------------------------------------------------------------------- */
+
+/* Canonical decoder (the reference's mlp1/mlp2 after convert.py:70-77's layer fusion):
+ *   z = F;  z = relu(W_i z + b_i) for n_pre layers;  sigma = act(w_s.z + b_s)
+ *   y = W_c z + W_v PE(d) + b_c;  y = W_j relu(y) + b_j for n_post layers
+ *   rgb = sigmoid(W_o relu(y) + b_o)
+ * Parameter layout (float32, row-major [out][in], Z = n_pre ? hidden : channels, P = 27):
+ *   pre 0: pre_identity ? b[H] : W[H][C], b[H];  pre i>0: W[H][H], b[H]
+ *   w_s[Z], b_s, W_c[H][Z], W_v[H][P], b_c[H];  post: W[H][H], b[H];  W_o[3][H], b_o[3]
+ * PE(d) = [d, sin(pi d), cos(pi d), ..., sin(8 pi d), cos(8 pi d)] (model/mlps.py:12-18). */
+typedef struct diver_decoder_desc {
+    int channels;
+    int hidden;
+    int n_pre;
+    int pre_identity;
+    int density_act; /* 0 = relu (model/diver.py:195), 1 = softplus */
+    int n_post;
+} diver_decoder_desc;
+
+typedef struct diver_decoder diver_decoder;
+
+int diver_decoder_create(const diver_decoder_desc* desc, const float* params_host,
+                         int64_t n_params, diver_decoder** out);
+int diver_decoder_destroy(diver_decoder* dec);
+int64_t diver_decoder_param_count(const diver_decoder_desc* desc);
+
+/* Real-time decode + composite of one march round.  Replaces mlp_eval(rgba, coord, voxels, v,
+ * mask) (utils/mlp_evaluation/include/mlp_eval.h:5-11, kernels
+ * utils/mlp_evaluation{,64}/src/mlp_eval.cu:32-231).  The reference's upload_weight
+ * (mlp_eval.h:13-17) installs the decoder and the corner map as process-global state; here they
+ * are explicit: `dec` plus `voxel_map` (device int32 [(map_size)^3], rows of `features`).
+ * features [rows][channels]; alpha_skip / t_stop are the reference's 1e-2 / 1e-2. */
+int diver_mlp_eval(const diver_decoder* dec, int64_t batch, const float* coord,
+                   const float* features, const int32_t* voxel_map, int map_size, const float* v,
+                   float* rgba, uint8_t* finish, float alpha_skip, float t_stop,
+                   diver_stream_t stream);
+
+/* ---- fused renderer --------------------------------------------------------------------------- */
+
+typedef struct diver_grid {
+    const float* voxels;  /* [(n+1)^3][channels] */
+    const uint8_t* mask;  /* [n^3] */
+    int n;
+    int channels;
+    float xyzmin;
+    float voxel_size;
+} diver_grid;
+
+enum diver_precision { DIVER_FP32 = 0, DIVER_BF16 = 1, DIVER_FP16 = 2 };
+
+/* Traversal + integral + decoder + compositing of explicit rays in one launch (the fused path
+ * of DIVeR.forward + DIVeR.render, model/diver.py:141-207, with front-to-back early
+ * termination at transmittance < t_stop).  rgba [n_rays][4] = (accumulated rgb, T); add T for
+ * a white background.  count [n_rays] (optional) = intervals composited. */
+int diver_render_rays(const diver_grid* grid, const diver_decoder* dec, int precision,
+                      int64_t n_rays, const float* o, const float* d, float t_stop, float* rgba,
+                      int32_t* count, diver_stream_t stream);
+
+/* Same for a pinhole camera (utils/dataset/ray_utils.py:5-56 conventions: pixel centres +0.5,
+ * -z forward, normalised directions); c2w_host is row-major 3x4.  rgba/count are [height][width]. */
+int diver_render_camera(const diver_grid* grid, const diver_decoder* dec, int precision,
+                        const float* c2w_host, int width, int height, float focal, float t_stop,
+                        float* rgba, int32_t* count, diver_stream_t stream);
+
+/* The FP32 rays diver_render_camera traces: o, d [height*width][3]. */
+int diver_camera_rays(const float* c2w_host, int width, int height, float focal, float* o,
+                      float* d, diver_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DIVER_H */

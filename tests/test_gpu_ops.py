@@ -1,0 +1,170 @@
+"""GPU parity of the segment integral, compositing, the real-time decoder (mlp_eval) and the
+DIVeR model contract against the FP64 oracle.
+
+Tolerances (FP32 kernels vs FP64): integral max-abs 2e-6 on features of scale 0.1; compositing
+max-abs 2e-6; mlp_eval rgba max-abs 2e-4 with thresholds off (every ray compared), and with the
+reference thresholds on, >= 99% of rays within 2e-4 (rays whose alpha or transmittance sits at a
+threshold may branch differently in FP32 and FP64).
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from oracle import fp64
+from diver_amd import calls
+from diver_amd import synthetic as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _intervals(sc, o, d):
+    from oracle.render import intervals
+    return intervals(o, d, sc.mask, sc.n, sc.xyzmin, sc.voxel_size)
+
+
+def test_integrate_matches_fp64(scene64, cameras, dev):
+    o, d = S.camera_rays(cameras[0], 64, 64)
+    ray, pin, pout, _ = _intervals(scene64, o, d)
+    vox = scene64.dense_voxels()
+    ref = fp64.integrate_einsum(vox, pin, pout)
+    got = calls.integrate(torch.from_numpy(vox).to(dev), torch.from_numpy(pin).to(dev),
+                          torch.from_numpy(pout).to(dev)).cpu().numpy()
+    assert np.abs(got - ref).max() < 2e-6
+
+
+def test_integrate_backward_matches_adjoint(scene64, cameras, dev):
+    """<g, I(v)> = <I^T g, v>: the scatter-add adjoint against the forward in FP64."""
+    o, d = S.camera_rays(cameras[1], 64, 64)
+    ray, pin, pout, _ = _intervals(scene64, o, d)
+    rng = np.random.default_rng(0)
+    g = rng.normal(size=(pin.shape[0], 32)).astype(np.float32)
+    shape = (65, 65, 65, 32)
+    gv = calls.integrate_backward(shape, torch.from_numpy(pin).to(dev),
+                                  torch.from_numpy(pout).to(dev),
+                                  torch.from_numpy(g).to(dev)).cpu().numpy().astype(np.float64)
+    v = rng.normal(size=shape)
+    lhs = np.sum(g.astype(np.float64) * fp64.integrate_einsum(v, pin, pout))
+    rhs = np.sum(gv * v)
+    assert abs(lhs - rhs) < 1e-4 * max(1.0, abs(lhs))
+
+
+def test_composite_forward_backward(dev):
+    rng = np.random.default_rng(1)
+    B, M = 257, 40
+    sigma = rng.normal(scale=2.0, size=(B, M)).astype(np.float32)
+    color = rng.uniform(size=(B, M, 3)).astype(np.float32)
+    mask = rng.uniform(size=(B, M)) < 0.8
+    rgb_ref, w_ref = fp64.composite(sigma, color, mask, True)
+    ts = torch.from_numpy(sigma).to(dev)
+    tc = torch.from_numpy(color).to(dev)
+    tm = torch.from_numpy(mask).to(dev)
+    rgb, w = calls.composite(ts, tc, tm, True)
+    assert np.abs(rgb.cpu().numpy() - rgb_ref).max() < 2e-6
+    assert np.abs(w.cpu().numpy() - w_ref).max() < 2e-6
+    # gradients against torch autograd in float64 on the CPU
+    gr = rng.normal(size=(B, 3))
+    gw = rng.normal(size=(B, M))
+    s64 = torch.tensor(sigma, dtype=torch.float64, requires_grad=True)
+    c64 = torch.tensor(color, dtype=torch.float64, requires_grad=True)
+    m64 = torch.tensor(mask)
+    a = 1 - torch.exp(-torch.relu(s64) * m64)
+    sh = torch.cat([torch.ones(B, 1, dtype=torch.float64), 1 - a[:, :-1]], 1)
+    ww = a * torch.cumprod(sh, -1)
+    rr = (ww[:, :, None] * c64).sum(1) + (1 - ww.sum(1, keepdim=True))
+    ((rr * torch.tensor(gr)).sum() + (ww * torch.tensor(gw)).sum()).backward()
+    gs, gc = calls.composite_backward(ts, tc, tm, True, torch.from_numpy(gr.astype(np.float32)).to(dev),
+                                      torch.from_numpy(gw.astype(np.float32)).to(dev))
+    assert np.abs(gs.cpu().numpy() - s64.grad.numpy()).max() < 1e-5
+    assert np.abs(gc.cpu().numpy() - c64.grad.numpy()).max() < 1e-5
+
+
+def _rt_setup(sc, hidden, dev):
+    sd = S.reference_decoder_state(32, hidden)
+    blob = fp64.rt_blob_from_state(sd, hidden)
+    vox = sc.dense_voxels().astype(np.float64)
+    if hidden == 32:
+        vox = vox @ sd['mlp1.0.weight'].astype(np.float64).T   # convert.py:149-150
+    vox32 = vox.astype(np.float32)
+    # sparse rows + corner map, as eval_rt.py:215-233 builds them
+    rows = vox32.reshape(-1, 32)[sc.corner_idx]
+    vmap = np.zeros(sc.n1 ** 3, np.int32)
+    vmap[sc.corner_idx] = np.arange(sc.corner_idx.size, dtype=np.int32)
+    return blob, vox32.astype(np.float64), rows, vmap.reshape(sc.n1, sc.n1, sc.n1)
+
+
+@pytest.mark.parametrize('hidden', [32, 64])
+@pytest.mark.parametrize('thresholds', [(0.0, 0.0), (1e-2, 1e-2)])
+def test_mlp_eval_matches_fp64(scene64, cameras, dev, hidden, thresholds):
+    from diver_amd.utils import mlp_evaluation, mlp_evaluation64
+    mod = mlp_evaluation if hidden == 32 else mlp_evaluation64
+    sc = scene64
+    blob, vox64, rows, vmap = _rt_setup(sc, hidden, dev)
+    mod.upload_weight(0, torch.from_numpy(blob.astype(np.float32)), torch.from_numpy(vmap))
+    o, d = S.camera_rays(cameras[5], 48, 48)
+    B = d.shape[0]
+    n = sc.n
+    centre = ((o[0] - np.float32(sc.xyzmin)) / np.float32(sc.voxel_size)).astype(np.float32)
+    octrees = oracle.octree_levels(sc.mask)
+    coords = torch.zeros((8, B, 6), dtype=torch.float32, device=dev)
+    finish = torch.zeros(B, dtype=torch.bool, device=dev)
+    rgba = torch.zeros((B, 4), dtype=torch.float32, device=dev)
+    dg = torch.from_numpy(d).to(dev)
+    calls.aabb_intersect(coords, dg, torch.from_numpy(centre).to(dev), finish, rgba,
+                         torch.from_numpy(octrees).to(dev), n)
+    rgba_ref = rgba.cpu().numpy().astype(np.float64)
+    fin_ref = finish.cpu().numpy().astype(np.uint8)
+    mg = torch.from_numpy(sc.mask).to(dev)
+    feats = torch.from_numpy(rows).to(dev)
+    a_skip, t_stop = thresholds
+    for _ in range(60):
+        if bool(finish.all()):
+            break
+        calls.ray_march(coords, dg, mg, finish, n)
+        c_np = coords.cpu().numpy()
+        fp64.mlp_eval_round(c_np, vox64, blob, hidden, d, rgba_ref, fin_ref, a_skip, t_stop)
+        mod.mlp_eval(rgba, coords, feats, dg, finish, alpha_skip=a_skip, t_stop=t_stop)
+    got = rgba.cpu().numpy()
+    err = np.abs(got - rgba_ref).max(1)
+    if thresholds == (0.0, 0.0):
+        assert err.max() < 2e-4
+        assert np.array_equal(finish.cpu().numpy().astype(np.uint8), fin_ref)
+    else:
+        assert np.mean(err < 2e-4) >= 0.99
+
+
+def test_diver_model_contract(scene64, cameras, dev):
+    """DIVeR.forward / render against the FP64 restatement of model/diver.py:141-207."""
+    from types import SimpleNamespace
+    from diver_amd.model.diver import DIVeR
+    hp = SimpleNamespace(voxel_num=64, voxel_dim=32, grid_size=2.0, mask_scale=1.0,
+                         white_back=True, fine=1, implicit=False, mlp_point=[32, 2, 33],
+                         mlp_view=[32, 1], dir_encode=4)
+    model = DIVeR(hp).to(dev)
+    sd = S.reference_decoder_state(32, 32)
+    with torch.no_grad():
+        model.voxels.copy_(torch.from_numpy(scene64.dense_voxels()))
+        model.voxel_mask.copy_(torch.from_numpy(scene64.mask))
+        for k, v in sd.items():
+            dict(model.named_parameters())[k].copy_(torch.from_numpy(v))
+    o, d = S.camera_rays(cameras[6], 32, 32)
+    color, sigma, mask, ts = model(torch.from_numpy(o).to(dev), torch.from_numpy(d).to(dev))
+    rgb, weight = model.render(color, sigma, mask)
+    # FP64 reference
+    ray, pin, pout, num = _intervals(scene64, o, d)
+    F = fp64.integrate_einsum(scene64.dense_voxels(), pin, pout)
+    s_, c_ = fp64.mlp_chain(sd, F, d[ray].astype(np.float64))
+    B, M = mask.shape
+    assert M == num.max()
+    m_ref = np.arange(M)[None, :] < num[:, None]
+    assert np.array_equal(mask.cpu().numpy(), m_ref)
+    sig = np.zeros((B, M)); sig[m_ref] = s_
+    col = np.zeros((B, M, 3)); col[m_ref] = c_
+    rgb_ref, w_ref = fp64.composite(sig, col, m_ref, True)
+    assert np.abs(sigma.detach().cpu().numpy() - sig).max() < 1e-4
+    assert np.abs(rgb.detach().cpu().numpy() - rgb_ref).max() < 1e-4
+    assert np.abs(weight.detach().cpu().numpy() - w_ref).max() < 1e-4
+    # gradients reach the grid and the decoder
+    (rgb.sum() + 1e-5 * torch.log1p(sigma ** 2 / 0.5).mean()).backward()
+    assert model.voxels.grad is not None and float(model.voxels.grad.abs().sum()) > 0
+    assert model.mlp1[0].weight.grad is not None

@@ -1,0 +1,181 @@
+"""Pins the FP64 oracle (oracle/fp64.py) by closed forms, invariants and independent forms.
+
+- integral: the einsum form of utils/integrator.py, the corner-weight form of mlp_eval.cu and
+  2-point Gauss-Legendre quadrature of the trilinear interpolant (exact for the cubic) agree;
+  p_in = p_out gives plain trilinear interpolation; a constant field integrates to itself and a
+  linear field to its value at the segment midpoint; corner weights sum to one.
+- decoders: the canonical packing (diver_amd.decoder) reproduces the reference's unfused
+  definitions (mlp chain, real-time blob, benchmark decoder) in FP64.
+- compositing: cumprod form == front-to-back recursion; weights + final T sum to one.
+"""
+import numpy as np
+import pytest
+
+from oracle import fp64
+
+
+def _segments(n, k, seed):
+    rng = np.random.default_rng(seed)
+    vox = rng.integers(0, n, size=(k, 3))
+    a = vox + rng.uniform(0, 1, size=(k, 3))
+    b = vox + rng.uniform(0, 1, size=(k, 3))
+    # put one coordinate of each endpoint on a face, as traversal output does
+    ax = rng.integers(0, 3, size=k)
+    a[np.arange(k), ax] = vox[np.arange(k), ax]
+    bx = rng.integers(0, 3, size=k)
+    b[np.arange(k), bx] = vox[np.arange(k), bx] + 1
+    return a.astype(np.float32), b.astype(np.float32)
+
+
+def test_three_integral_forms_agree():
+    rng = np.random.default_rng(0)
+    n, C = 6, 5
+    vox = rng.normal(size=(n + 1, n + 1, n + 1, C))
+    a, b = _segments(n, 500, 1)
+    e = fp64.integrate_einsum(vox, a, b)
+    w = fp64.integrate_weights(vox, a, b)
+    g = fp64.integrate_gauss(vox, a, b)
+    assert np.abs(e - g).max() < 1e-12
+    assert np.abs(w - g).max() < 1e-12
+
+
+def test_integral_special_cases():
+    rng = np.random.default_rng(2)
+    n, C = 4, 3
+    a, b = _segments(n, 200, 3)
+    const = np.ones((n + 1, n + 1, n + 1, C)) * np.array([1.5, -2.0, 0.25])
+    assert np.allclose(fp64.integrate_einsum(const, a, b), const[0, 0, 0])
+    # linear field f(x,y,z) = c0 + c . p : mean over the segment = value at the midpoint
+    zz, yy, xx = np.meshgrid(*[np.arange(n + 1)] * 3, indexing='ij')
+    coef = np.array([0.3, -1.1, 2.0])
+    lin = (0.7 + coef[0] * xx + coef[1] * yy + coef[2] * zz)[..., None]
+    mid = 0.5 * (a.astype(np.float64) + b.astype(np.float64))
+    assert np.allclose(fp64.integrate_einsum(lin, a, b)[:, 0], 0.7 + mid @ coef, atol=1e-12)
+    # zero-length segment: plain trilinear interpolation
+    vox = rng.normal(size=(n + 1, n + 1, n + 1, C))
+    pmin = fp64.voxel_corner(a, a)
+    assert np.allclose(fp64.integrate_einsum(vox, a, a), fp64.trilerp(vox, pmin, a.astype(np.float64)))
+    w, _ = fp64.corner_weights(a, b)
+    assert np.allclose(w.sum(1), 1.0)
+
+
+def test_integral_upper_corner_clamped():
+    # segments on the far face: the upper corner index is clamped to N (integrator.py:79)
+    n, C = 3, 2
+    vox = np.random.default_rng(4).normal(size=(n + 1, n + 1, n + 1, C))
+    a = np.array([[3.0, 1.2, 0.5]], np.float32)
+    b = np.array([[3.0, 1.7, 0.9]], np.float32)
+    e = fp64.integrate_einsum(vox, a, b)
+    assert np.all(np.isfinite(e))
+    assert np.allclose(e, fp64.integrate_weights(vox, a, b))
+
+
+def _rows(k, C, seed):
+    rng = np.random.default_rng(seed)
+    f = rng.normal(scale=0.3, size=(k, C))
+    d = rng.normal(size=(k, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return f, d
+
+
+def canonical_eval(canon, f, d):
+    """FP64 evaluation of the canonical decoder form (include/diver.h)."""
+    L = canon.layers()
+    z = f
+    for i, (w, b) in enumerate(L['pre']):
+        z = np.maximum((z if w is None else z @ w.astype(np.float64).T) + b, 0.0)
+    s = z @ L['w_s'].astype(np.float64) + L['b_s']
+    y = z @ L['W_c'].astype(np.float64).T + fp64.positional_encoding(d) @ L['W_v'].astype(np.float64).T + L['b_c']
+    for (w, b) in L['post']:
+        y = np.maximum(y, 0) @ w.astype(np.float64).T + b
+    o = np.maximum(y, 0) @ L['W_o'].astype(np.float64).T + L['b_o']
+    return s, 1 / (1 + np.exp(-o))
+
+
+@pytest.mark.parametrize('hidden', [32, 64])
+def test_canonical_from_state_dict(hidden):
+    from diver_amd import synthetic as S
+    from diver_amd.decoder import from_state_dict
+    sd = S.reference_decoder_state(32, hidden)
+    canon = from_state_dict(sd, 32)
+    f, d = _rows(300, 32, 7)
+    s_ref, c_ref = fp64.mlp_chain(sd, f, d)
+    s, c = canonical_eval(canon, f, d)
+    assert np.abs(s - s_ref).max() < 1e-4
+    assert np.abs(c - c_ref).max() < 1e-5
+
+
+@pytest.mark.parametrize('hidden', [32, 64])
+def test_canonical_from_rt_blob(hidden):
+    """Blob built by the reference's own algebra (convert.py:63-140), restated here in numpy."""
+    from diver_amd import synthetic as S
+    from diver_amd.decoder import from_rt_blob
+    sd = {k: v.astype(np.float64) for k, v in S.reference_decoder_state(32, hidden).items()}
+    perm = np.array([0, 3, 9, 15, 21, 6, 12, 18, 24, 1, 4, 10, 16, 22, 7, 13, 19, 25,
+                     2, 5, 11, 17, 23, 8, 14, 20, 26])
+    H = hidden
+    last = 'mlp1.4' if H == 32 else 'mlp1.2'
+    l21, b21 = sd[last + '.weight'], sd[last + '.bias']
+    l22, b22 = sd['mlp2.0.weight'], sd['mlp2.0.bias']
+    l40, b40 = l21[:1], b21[:1]
+    l41 = l22[:, :H] @ l21[1:]
+    b41 = l22[:, :H] @ b21[1:] + b22
+    l42 = l22[:, H:]
+    l4 = np.concatenate([l40, l41], 0)
+    b4 = np.concatenate([b40, b41], 0)
+    w1 = np.concatenate([b4[0:1], l4[0]], 0)
+    w2 = np.concatenate([b4[1:, None], l4[1:], l42], 1)
+    w2 = np.concatenate([w2[:, :-27], w2[:, -27:][:, perm]], -1)
+    w3 = np.concatenate([sd['mlp2.2.bias'][:, None], sd['mlp2.2.weight']], -1)
+    if H == 64:
+        w0 = np.concatenate([sd['mlp1.0.bias'][:, None], sd['mlp1.0.weight']], 1)
+        blob = np.concatenate([w0.T.reshape(-1), w1, w2.T.reshape(-1), w3.T.reshape(-1),
+                               np.zeros(28)])
+        assert blob.size == 8288
+    else:
+        win = np.concatenate([sd['mlp1.0.bias'][:, None], sd['mlp1.0.weight']], 1)
+        w0 = np.concatenate([sd['mlp1.2.bias'][:, None], sd['mlp1.2.weight']], 1)
+        blob = np.concatenate([win[:, 0], w0.T.reshape(-1), w1, w2.T.reshape(-1),
+                               w3.T.reshape(-1), np.zeros(28)])
+        assert blob.size == 3168
+    f, d = _rows(300, 32, 8)
+    s_ref, c_ref = fp64.mlp_chain(sd, f, d)
+    feat_rt = f @ sd['mlp1.0.weight'].T if H == 32 else f   # convert.py:149-150
+    s_blob, c_blob = fp64.rt_blob_decoder(blob, H, feat_rt, d)
+    assert np.abs(s_blob - s_ref).max() < 1e-9 and np.abs(c_blob - c_ref).max() < 1e-9
+    canon = from_rt_blob(blob, H)
+    s, c = canonical_eval(canon, feat_rt, d)
+    assert np.abs(s - s_ref).max() < 1e-4 and np.abs(c - c_ref).max() < 1e-5
+
+
+@pytest.mark.parametrize('hidden,channels', [(32, 32), (64, 32), (64, 64)])
+def test_canonical_baseline(hidden, channels):
+    from diver_amd import synthetic as S
+    from diver_amd.decoder import from_baseline
+    layers = S.baseline_decoder_layers(channels, hidden)
+    canon = from_baseline(layers, channels)
+    f, d = _rows(300, channels, 9)
+    s_ref, c_ref = fp64.baseline_decoder(layers, f, d)
+    s, c = canonical_eval(canon, f, d)
+    assert np.abs(s - s_ref).max() < 1e-6 and np.abs(c - c_ref).max() < 1e-5
+
+
+def test_composite_forms_agree():
+    rng = np.random.default_rng(10)
+    B, M = 50, 30
+    sigma = rng.normal(scale=1.0, size=(B, M))
+    color = rng.uniform(size=(B, M, 3))
+    mask = rng.uniform(size=(B, M)) < 0.7
+    rgb, weight = fp64.composite(sigma, color, mask, True)
+    # front-to-back recursion over the masked rows
+    idx, alpha, col = [], [], []
+    for b in range(B):
+        for m in range(M):
+            if mask[b, m]:
+                idx.append(b)
+                alpha.append(1 - np.exp(-max(sigma[b, m], 0)))
+                col.append(color[b, m])
+    acc, T, _ = fp64.render_front_to_back(np.array(idx), np.array(alpha), np.array(col), B,
+                                          t_stop=-1.0)
+    assert np.allclose(rgb, acc + T[:, None])
+    assert np.allclose(weight.sum(1) + T, 1.0)

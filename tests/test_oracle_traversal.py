@@ -1,0 +1,230 @@
+"""Pins the CPU traversal oracle (oracle/traverse.c) against what the geometry fixes.
+
+The reference has no tests or golden vectors, so the FP32 restatement of its DDA is checked here
+against properties any correct traversal of [0,N]^3 must have and against brute force:
+- each emitted interval lies in the voxel it is attributed to, consecutive intervals of the
+  unmasked traversal share endpoints, endpoints lie on voxel faces;
+- the voxel sequence equals the one found by dense sampling along the ray (exact FP64 geometry)
+  on small grids, for rays that stay away from voxel edges;
+- masked traversal = unmasked traversal filtered by the mask;
+- axis-aligned rays, rays from inside the grid and misses behave as the reference's code
+  dictates (misses give no entries; a +0.0 direction component stops the ray after entry).
+A planted mistake (a wrong snap axis or a dropped 1e-4 nudge) breaks the brute-force check.
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+
+def rays_through_box(n, count, seed, inside_frac=0.0):
+    rng = np.random.default_rng(seed)
+    tgt = rng.uniform(0.1, n - 0.1, size=(count, 3))
+    dirs = rng.normal(size=(count, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    o = tgt - dirs * rng.uniform(n * 0.6, n * 2.0, size=(count, 1))
+    k = int(count * inside_frac)
+    if k:
+        o[:k] = rng.uniform(0.5, n - 0.5, size=(k, 3))
+    return o.astype(np.float32), dirs.astype(np.float32)
+
+
+def brute_voxels(o, d, n, samples=200000):
+    """Voxel sequence along the ray inside [0,n]^3 by dense FP64 sampling."""
+    o = o.astype(np.float64)
+    d = d.astype(np.float64)
+    with np.errstate(divide='ignore', invalid='ignore'):
+        t0 = (0 - o) / d
+        t1 = (n - o) / d
+    tmin = np.nanmax(np.minimum(t0, t1))
+    tmax = np.nanmin(np.maximum(t0, t1))
+    if tmin > tmax:
+        return []
+    tmin = max(tmin, 0.0)
+    ts = np.linspace(tmin, tmax, samples)[1:-1]
+    p = o[None] + ts[:, None] * d[None]
+    v = np.floor(p).astype(np.int64)
+    v = np.clip(v, 0, n - 1)
+    keep = np.ones(len(v), bool)
+    keep[1:] = np.any(v[1:] != v[:-1], axis=1)
+    return [tuple(x) for x in v[keep]]
+
+
+def edge_distance(o, d, n):
+    """Smallest distance (voxel units) of the segment's plane crossings to a voxel edge."""
+    o = o.astype(np.float64)
+    d = d.astype(np.float64)
+    best = np.inf
+    for a in range(3):
+        if d[a] == 0:
+            continue
+        for k in range(0, n + 1):
+            t = (k - o[a]) / d[a]
+            p = o + t * d
+            if np.any(p < 0) or np.any(p > n):
+                continue
+            for b in range(3):
+                if b == a:
+                    continue
+                best = min(best, abs(p[b] - np.round(p[b])))
+    return best
+
+
+def test_unmasked_chain_and_faces():
+    n = 16
+    o, d = rays_through_box(n, 300, 1, inside_frac=0.2)
+    isect, num, tns = oracle.ray_voxel_intersect(o, d, 0.0, float(n), n, 1.0)
+    assert (num <= 3 * n).all()
+    for b in range(len(o)):
+        k = num[b]
+        if k == 0:
+            continue
+        pts = isect[b, :k]
+        assert np.all(pts >= 0) and np.all(pts <= n)
+        # every point after the first lies on at least one integer plane
+        if k > 1:
+            frac = np.abs(pts[1:] - np.round(pts[1:]))
+            assert np.all(frac.min(axis=1) == 0)
+        # distances are non-decreasing
+        assert np.all(np.diff(tns[b, :k]) >= -1e-6)
+        assert np.all(tns[b, k:] == -1)
+
+
+@pytest.mark.parametrize('n', [4, 8])
+def test_voxel_sequence_matches_brute_force(n):
+    o, d = rays_through_box(n, 400, 2 + n)
+    mask = np.ones((n, n, n), np.uint8)
+    isect, num, _ = oracle.masked_intersect(o, d, 0.0, float(n), n, 1.0, mask, 1.0)
+    checked = 0
+    for b in range(len(o)):
+        if edge_distance(o[b], d[b], n) < 1e-3:
+            continue
+        ref = brute_voxels(o[b], d[b], n)
+        got = []
+        for k in range(num[b]):
+            p0, p1 = isect[b, k, :3], isect[b, k, 3:]
+            v = np.minimum((p0 + np.float32(1e-4)).astype(np.int64),
+                           (p1 + np.float32(1e-4)).astype(np.int64))
+            got.append((v[2], v[1], v[0]))  # brute force works in (x,y,z) of p; compare below
+        got_xyz = [(g[2], g[1], g[0]) for g in got]
+        assert got_xyz == ref, (b, got_xyz[:6], ref[:6])
+        checked += 1
+    assert checked > 300
+
+
+def test_masked_is_filtered_unmasked():
+    n = 16
+    rng = np.random.default_rng(5)
+    mask = (rng.uniform(size=(n, n, n)) < 0.3).astype(np.uint8)
+    o, d = rays_through_box(n, 500, 6, inside_frac=0.1)
+    full, numf, tf = oracle.masked_intersect(o, d, 0.0, float(n), n, 1.0, np.ones_like(mask), 1.0)
+    part, nump, tp = oracle.masked_intersect(o, d, 0.0, float(n), n, 1.0, mask, 1.0)
+    for b in range(len(o)):
+        rows = []
+        for k in range(numf[b]):
+            p0, p1 = full[b, k, :3], full[b, k, 3:]
+            v = np.minimum((p0 + np.float32(1e-4)).astype(np.int64),
+                           (p1 + np.float32(1e-4)).astype(np.int64))
+            if mask[v[2], v[1], v[0]]:
+                rows.append((full[b, k], tf[b, k]))
+        assert len(rows) == nump[b]
+        for k, (r, t) in enumerate(rows):
+            assert np.array_equal(r, part[b, k]) and t == tp[b, k]
+
+
+def test_masked_consecutive_share_endpoints():
+    n = 32
+    o, d = rays_through_box(n, 200, 9)
+    isect, num, _ = oracle.masked_intersect(o, d, 0.0, float(n), n, 1.0,
+                                            np.ones((n, n, n), np.uint8), 1.0)
+    for b in range(len(o)):
+        k = num[b]
+        if k > 1:
+            assert np.array_equal(isect[b, 1:k, :3], isect[b, :k - 1, 3:])
+
+
+def test_axis_aligned_and_special_rays():
+    n = 8
+    mask = np.ones((n, n, n), np.uint8)
+    # straight along +x through voxel centres: exactly n intervals
+    o = np.array([[-3.0, 2.5, 4.5]], np.float32)
+    d = np.array([[1.0, -0.0, -0.0]], np.float32)
+    isect, num, tns = oracle.masked_intersect(o, d, 0.0, float(n), n, 1.0, mask, 1.0)
+    assert num[0] == n
+    assert np.allclose(isect[0, :n, 0], np.arange(n)) and np.allclose(isect[0, :n, 3], np.arange(1, n + 1))
+    assert np.allclose(tns[0, :n], 3.0 + np.arange(1, n + 1))
+    # +0.0 components make t = -inf on that axis: the reference stops right after entry
+    d2 = np.array([[1.0, 0.0, 0.0]], np.float32)
+    _, num2, _ = oracle.masked_intersect(o, d2, 0.0, float(n), n, 1.0, mask, 1.0)
+    assert num2[0] == 0
+    # a miss
+    o3 = np.array([[-3.0, 20.0, 4.5]], np.float32)
+    _, num3, tns3 = oracle.masked_intersect(o3, d, 0.0, float(n), n, 1.0, mask, 1.0)
+    assert num3[0] == 0 and np.all(tns3 == -1)
+    # origin inside: first interval starts at the origin, t measured from it
+    o4 = np.array([[2.25, 2.5, 4.5]], np.float32)
+    isect4, num4, tns4 = oracle.masked_intersect(o4, d, 0.0, float(n), n, 1.0, mask, 1.0)
+    assert num4[0] == n - 2 and isect4[0, 0, 0] == np.float32(2.25)
+    assert np.isclose(tns4[0, 0], 0.75)
+    # a zero direction terminates (bounded loop) instead of spinning
+    _, num5, _ = oracle.masked_intersect(o4, np.zeros((1, 3), np.float32), 0.0, float(n), n, 1.0,
+                                         mask, 1.0)
+    assert num5[0] >= 0
+
+
+def test_world_units_and_mask_scale():
+    n = 16
+    rng = np.random.default_rng(11)
+    coarse = (rng.uniform(size=(8, 8, 8)) < 0.4).astype(np.uint8)
+    fine = np.repeat(np.repeat(np.repeat(coarse, 2, 0), 2, 1), 2, 2)
+    o, d = rays_through_box(n, 300, 12)
+    # world units: grid over [-1, 1]^3
+    vs = 2.0 / n
+    ow = (o * vs - 1.0).astype(np.float32)
+    a, na, ta = oracle.masked_intersect(ow, d, -1.0, 1.0, n, vs, coarse, 0.5)
+    b, nb, tb = oracle.masked_intersect(ow, d, -1.0, 1.0, n, vs, fine, 1.0)
+    # a half-resolution mask selects a superset-consistent subset: counts agree where the
+    # nudged corner maps identically (almost everywhere)
+    assert np.mean(na == nb) > 0.97
+
+
+def test_real_time_march_covers_masked_traversal(scene64, cameras):
+    """aabb_intersect + ray_march rounds emit the same occupied intervals as masked_intersect
+    from the first occupied N/4 block on (ray_march.cu / ray_voxel_intersect.cu share the DDA)."""
+    from diver_amd import synthetic as S
+    sc = scene64
+    n = sc.n
+    o, d = S.camera_rays(cameras[0], 32, 32)
+    B = o.shape[0]
+    centre = ((o[0] - np.float32(sc.xyzmin)) / np.float32(sc.voxel_size)).astype(np.float32)
+    octrees = oracle.octree_levels(sc.mask)
+    coords = np.zeros((8, B, 6), np.float32)
+    finish = np.zeros(B, np.uint8)
+    rgba = np.zeros((B, 4), np.float32)
+    oracle.aabb_intersect(coords, d, centre, finish, rgba, octrees, n)
+    mask = sc.mask.astype(np.uint8)
+    rows = [[] for _ in range(B)]
+    fin = finish.copy()
+    for _ in range(64):
+        if fin.all():
+            break
+        oracle.ray_march(coords, d, mask, fin, n)
+        for b in range(B):
+            if fin[b]:
+                continue
+            for h in range(8):
+                if coords[h, b, 0] < 0:
+                    fin[b] = 1
+                    break
+                rows[b].append(coords[h, b].copy())
+    assert fin.all()
+    ref, num, _ = oracle.masked_intersect(np.repeat(o[:1], B, 0), d, np.float32(sc.xyzmin), 1.0, n,
+                                          np.float32(sc.voxel_size), mask, 1.0)
+    agree = 0
+    for b in range(B):
+        got = np.array(rows[b]).reshape(-1, 6)
+        if len(got) == num[b] and (len(got) == 0 or np.allclose(got, ref[b, :num[b]], atol=2e-4)):
+            agree += 1
+    # the real-time path starts at a block entry (re-derived point) and stops at >= N, so a few
+    # rays differ in the last bits or the last interval
+    assert agree >= 0.9 * B
