@@ -138,11 +138,12 @@ def main():
         rgb32, _, _ = render_camera(grid, dec, cam, W, H, precision='fp32', t_stop=t_stop)
         rgb16 = rgb16.reshape(-1, 3)[sel].cpu().numpy()
         rgb32 = rgb32.reshape(-1, 3)[sel].cpu().numpy()
+        img_plain, _ = rt.render(cam, W, H)
         img_p = img_plain.reshape(-1, 3)[sel].cpu().numpy()
         vox_np = sc.dense_voxels()
         ref = oracle_render(vox_np, sc.mask, sc.n, sc.xyzmin, sc.voxel_size,
                             lambda F, dd: fp64.baseline_decoder(layers, F, dd), 1, o_s, d_s,
-                            t_stop)
+                            t_stop, walk='realtime')
         rgb_ref = ref['rgb'] + ref['T'][:, None]
 
         def psnr(a):

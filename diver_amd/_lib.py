@@ -39,9 +39,15 @@ class DecoderDesc(ctypes.Structure):
 
 
 class Grid(ctypes.Structure):
-    _fields_ = [('voxels', ctypes.c_void_p), ('mask', ctypes.c_void_p), ('n', ctypes.c_int),
+    _fields_ = [('voxels', ctypes.c_void_p), ('mask', ctypes.c_void_p),
+                ('octrees', ctypes.c_void_p), ('n', ctypes.c_int),
                 ('channels', ctypes.c_int), ('xyzmin', ctypes.c_float),
                 ('voxel_size', ctypes.c_float)]
+
+
+class RenderOpts(ctypes.Structure):
+    _fields_ = [('precision', ctypes.c_int), ('traversal', ctypes.c_int),
+                ('t_stop', ctypes.c_float), ('alpha_skip', ctypes.c_float)]
 
 
 def _declare(L):
@@ -63,9 +69,10 @@ def _declare(L):
                                  ctypes.POINTER(vp)],
         'diver_decoder_destroy': [vp],
         'diver_mlp_eval': [vp, i64, vp, vp, vp, i32, vp, vp, vp, fl, fl, vp],
-        'diver_render_rays': [ctypes.POINTER(Grid), vp, i32, i64, vp, vp, fl, vp, vp, vp],
-        'diver_render_camera': [ctypes.POINTER(Grid), vp, i32, ctypes.POINTER(ctypes.c_float), i32,
-                                i32, fl, fl, vp, vp, vp],
+        'diver_render_rays': [ctypes.POINTER(Grid), vp, ctypes.POINTER(RenderOpts), i64, vp, vp,
+                              vp, vp, vp],
+        'diver_render_camera': [ctypes.POINTER(Grid), vp, ctypes.POINTER(RenderOpts),
+                                ctypes.POINTER(ctypes.c_float), i32, i32, fl, vp, vp, vp],
         'diver_camera_rays': [ctypes.POINTER(ctypes.c_float), i32, i32, fl, vp, vp, vp],
     }
     for name, args in sig.items():

@@ -172,40 +172,44 @@ def mlp_eval(decoder, rgba, coord, features, voxel_map, v, finish, alpha_skip, t
 
 
 PRECISIONS = {'fp32': 0, 'bf16': 1, 'fp16': 2}
+TRAVERSALS = {'offline': 0, 'realtime': 1}
 
 
-def grid_struct(voxels, mask, xyzmin, voxel_size):
+def grid_struct(voxels, mask, octrees, xyzmin, voxel_size):
     n = mask.shape[0]
-    return _lib.Grid(voxels.data_ptr(), mask.data_ptr(), int(n), int(voxels.shape[-1]),
-                     float(xyzmin), float(voxel_size))
+    return _lib.Grid(voxels.data_ptr(), mask.data_ptr(),
+                     octrees.data_ptr() if octrees is not None else None, int(n),
+                     int(voxels.shape[-1]), float(xyzmin), float(voxel_size))
 
 
-def render_rays(voxels, mask, xyzmin, voxel_size, decoder, o, d, precision='fp32', t_stop=1e-4,
-                with_count=True):
-    dev = _dev(voxels, mask, o, d)
-    voxels, mask = _f32(voxels, 'voxels'), _u8(mask, 'mask')
+def render_opts(precision='fp32', traversal='realtime', t_stop=1e-4, alpha_skip=0.0):
+    return _lib.RenderOpts(PRECISIONS[precision], TRAVERSALS[traversal], float(t_stop),
+                           float(alpha_skip))
+
+
+def render_rays(grid, decoder, o, d, opts, with_count=True):
+    """grid: an object with voxels, mask, octrees, xyzmin, voxel_size (renderer.SceneGrid)."""
+    dev = _dev(grid.voxels, grid.mask, o, d)
     o, d = _f32(o, 'o'), _f32(d, 'd')
     R = o.shape[0]
     rgba = torch.empty((R, 4), dtype=torch.float32, device=dev)
     count = torch.empty(R, dtype=torch.int32, device=dev) if with_count else None
-    g = grid_struct(voxels, mask, xyzmin, voxel_size)
-    _lib.call('diver_render_rays', ctypes.byref(g), decoder.handle, PRECISIONS[precision], R,
-              o.data_ptr(), d.data_ptr(), float(t_stop), rgba.data_ptr(), _lib.ptr(count), _s(o))
+    g = grid_struct(grid.voxels, grid.mask, grid.octrees, grid.xyzmin, grid.voxel_size)
+    _lib.call('diver_render_rays', ctypes.byref(g), decoder.handle, ctypes.byref(opts), R,
+              o.data_ptr(), d.data_ptr(), rgba.data_ptr(), _lib.ptr(count), _s(o))
     return rgba, count
 
 
-def render_camera(voxels, mask, xyzmin, voxel_size, decoder, c2w, width, height, focal,
-                  precision='fp32', t_stop=1e-4, with_count=True, out=None):
-    dev = _dev(voxels, mask)
-    voxels, mask = _f32(voxels, 'voxels'), _u8(mask, 'mask')
+def render_camera(grid, decoder, c2w, width, height, focal, opts, with_count=True, out=None):
+    dev = _dev(grid.voxels, grid.mask)
     rgba = out if out is not None else torch.empty((height, width, 4), dtype=torch.float32,
                                                    device=dev)
     count = torch.empty((height, width), dtype=torch.int32, device=dev) if with_count else None
-    g = grid_struct(voxels, mask, xyzmin, voxel_size)
+    g = grid_struct(grid.voxels, grid.mask, grid.octrees, grid.xyzmin, grid.voxel_size)
     m = (ctypes.c_float * 12)(*[float(x) for x in list(c2w.reshape(-1))])
-    _lib.call('diver_render_camera', ctypes.byref(g), decoder.handle, PRECISIONS[precision], m,
-              int(width), int(height), float(focal), float(t_stop), rgba.data_ptr(),
-              _lib.ptr(count), _s(voxels))
+    _lib.call('diver_render_camera', ctypes.byref(g), decoder.handle, ctypes.byref(opts), m,
+              int(width), int(height), float(focal), rgba.data_ptr(), _lib.ptr(count),
+              _s(grid.voxels))
     return rgba, count
 
 

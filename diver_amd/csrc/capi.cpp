@@ -295,14 +295,21 @@ int diver_mlp_eval(const diver_decoder* dec, int64_t batch, const float* coord,
                       "mlp_eval");
 }
 
-static int render_common(const diver_grid* g, const diver_decoder* dec, int precision,
-                         diver::RenderArgs& a) {
-    REQUIRE(g && dec, "null grid or decoder");
+static int render_common(const diver_grid* g, const diver_decoder* dec,
+                         const diver_render_opts* opts, diver::RenderArgs& a) {
+    REQUIRE(g && dec && opts, "null grid, decoder or options");
     REQUIRE(g->voxels && g->mask, "null grid arrays");
     REQUIRE(g->n > 0 && g->voxel_size > 0, "bad grid size");
     REQUIRE(g->channels == dec->desc.channels, "grid has %d channels, decoder expects %d",
             g->channels, dec->desc.channels);
+    const int precision = opts->precision;
     REQUIRE(precision >= DIVER_FP32 && precision <= DIVER_FP16, "bad precision %d", precision);
+    REQUIRE(opts->traversal == DIVER_WALK_OFFLINE || opts->traversal == DIVER_WALK_REALTIME,
+            "bad traversal %d", opts->traversal);
+    if (opts->traversal == DIVER_WALK_REALTIME) {
+        REQUIRE(g->octrees, "the real-time walk needs the occupancy pyramid");
+        REQUIRE(g->n >= 64 && g->n % 64 == 0, "the real-time walk needs n to be a multiple of 64");
+    }
     if (dec->variant == 5)
         return fail(DIVER_EUNSUPPORTED,
                     "the real-time blob decoder (pre-multiplied features) is for mlp_eval only");
@@ -311,20 +318,24 @@ static int render_common(const diver_grid* g, const diver_decoder* dec, int prec
                     "the matrix-core decoder path supports decoders without pre-layers only");
     a.voxels = g->voxels;
     a.mask = g->mask;
+    a.octrees = g->octrees;
+    a.traversal = opts->traversal;
     a.n = g->n;
     a.channels = g->channels;
     a.xyzmin = g->xyzmin;
     a.voxel_size = g->voxel_size;
     a.W = dec->W;
     a.pack = precision == DIVER_BF16 ? dec->pack_bf16 : dec->pack_f16;
+    a.t_stop = opts->t_stop;
+    a.alpha_skip = opts->alpha_skip;
     return DIVER_OK;
 }
 
-int diver_render_rays(const diver_grid* grid, const diver_decoder* dec, int precision,
-                      int64_t n_rays, const float* o, const float* d, float t_stop, float* rgba,
-                      int32_t* count, diver_stream_t stream) {
+int diver_render_rays(const diver_grid* grid, const diver_decoder* dec,
+                      const diver_render_opts* opts, int64_t n_rays, const float* o,
+                      const float* d, float* rgba, int32_t* count, diver_stream_t stream) {
     diver::RenderArgs a{};
-    int st = render_common(grid, dec, precision, a);
+    int st = render_common(grid, dec, opts, a);
     if (st != DIVER_OK) return st;
     REQUIRE(n_rays >= 0, "n_rays < 0");
     REQUIRE(n_rays == 0 || (o && d && rgba), "null pointer");
@@ -332,17 +343,18 @@ int diver_render_rays(const diver_grid* grid, const diver_decoder* dec, int prec
     a.o = o;
     a.d = d;
     a.cam.width = 0;
-    a.t_stop = t_stop;
     a.rgba = rgba;
     a.count = count;
-    return hip_status(diver::launch_render(dec->variant, a, precision, S(stream)), "render_rays");
+    return hip_status(diver::launch_render(dec->variant, a, opts->precision, S(stream)),
+                      "render_rays");
 }
 
-int diver_render_camera(const diver_grid* grid, const diver_decoder* dec, int precision,
-                        const float* c2w_host, int width, int height, float focal, float t_stop,
-                        float* rgba, int32_t* count, diver_stream_t stream) {
+int diver_render_camera(const diver_grid* grid, const diver_decoder* dec,
+                        const diver_render_opts* opts, const float* c2w_host, int width,
+                        int height, float focal, float* rgba, int32_t* count,
+                        diver_stream_t stream) {
     diver::RenderArgs a{};
-    int st = render_common(grid, dec, precision, a);
+    int st = render_common(grid, dec, opts, a);
     if (st != DIVER_OK) return st;
     REQUIRE(c2w_host && rgba, "null pointer");
     REQUIRE(width > 0 && height > 0 && focal > 0, "bad camera");
@@ -352,10 +364,10 @@ int diver_render_camera(const diver_grid* grid, const diver_decoder* dec, int pr
     a.cam.focal = focal;
     int64_t tiles = (int64_t)((width + 7) / 8) * ((height + 7) / 8);
     a.n_rays = tiles * 64;
-    a.t_stop = t_stop;
     a.rgba = rgba;
     a.count = count;
-    return hip_status(diver::launch_render(dec->variant, a, precision, S(stream)), "render_camera");
+    return hip_status(diver::launch_render(dec->variant, a, opts->precision, S(stream)),
+                      "render_camera");
 }
 
 int diver_camera_rays(const float* c2w_host, int width, int height, float focal, float* o,

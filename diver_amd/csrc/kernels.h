@@ -15,6 +15,8 @@ struct CameraDesc {
 struct RenderArgs {
     const float* voxels;   // [(N+1)^3][C] fp32
     const uint8_t* mask;   // [N^3]
+    const uint8_t* octrees;  // occupancy pyramid (block sizes 64, 16, 4), real-time traversal
+    int traversal;         // 0: masked_intersect walk (exit > N); 1: real-time walk
     int n;
     int channels;
     float xyzmin;
@@ -26,6 +28,7 @@ struct RenderArgs {
     const float* d;
     CameraDesc cam;
     float t_stop;
+    float alpha_skip;
     float* rgba;           // [rays][4]: accumulated rgb (no background) and transmittance
     int32_t* count;        // [rays] intervals consumed (optional)
 };

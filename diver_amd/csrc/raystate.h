@@ -1,0 +1,240 @@
+// Per-ray traversal state shared by the fused forward and backward kernels, so the backward's
+// recompute walks exactly the intervals the forward composited.
+#pragma once
+#include "common.h"
+#include "kernels.h"
+
+namespace diver {
+
+struct RayWalk {
+    float lx, ly, lz;  // current point (grid units)
+    float dx, dy, dz;
+    float sx, sy, sz;
+    int steps;
+    int max_steps;
+    bool alive;
+
+    // World-space ray -> grid units, box entry as masked_intersect does it.
+    __device__ __forceinline__ void init(float ox, float oy, float oz, float dx_, float dy_,
+                                         float dz_, float xyzmin, float voxel_size, float n,
+                                         bool valid) {
+#pragma clang fp contract(off)
+        dx = dx_;
+        dy = dy_;
+        dz = dz_;
+        float gx = (ox - xyzmin) / voxel_size;
+        float gy = (oy - xyzmin) / voxel_size;
+        float gz = (oz - xyzmin) / voxel_size;
+        lx = gx;
+        ly = gy;
+        lz = gz;
+        bool inside = (gx >= 0) & (gy >= 0) & (gz >= 0) & (gx <= n) & (gy <= n) & (gz <= n);
+        alive = valid;
+        if (valid && !inside) {
+            float tmin;
+            alive = box_entry(gx, gy, gz, dx, dy, dz, n, lx, ly, lz, tmin);
+        }
+        sx = (float)sgnf(dx);
+        sy = (float)sgnf(dy);
+        sz = (float)sgnf(dz);
+        steps = 0;
+        max_steps = max_dda_steps(n);
+    }
+
+    // Real-time entry (aabb_intersect, ray_march.cu:50-148): box entry, then a DDA over the
+    // occupancy pyramid (levels N/64, N/16, N/4, coarse first), descending x4 at each occupied
+    // cell; the walk starts at the fine-level entry of the first occupied N/4 cell, or the ray is
+    // dead.  Bit-identical to aabb_intersect's start point.
+    __device__ __forceinline__ void init_rt(float ox, float oy, float oz, float dx_, float dy_,
+                                            float dz_, float xyzmin, float voxel_size, int n,
+                                            const uint8_t* __restrict__ octrees, bool valid) {
+#pragma clang fp contract(off)
+        const float nf = (float)n;
+        dx = dx_;
+        dy = dy_;
+        dz = dz_;
+        sx = (float)sgnf(dx);
+        sy = (float)sgnf(dy);
+        sz = (float)sgnf(dz);
+        steps = 0;
+        max_steps = max_dda_steps(nf);
+        alive = false;
+        if (!valid) return;
+        float px = (ox - xyzmin) / voxel_size;
+        float py = (oy - xyzmin) / voxel_size;
+        float pz = (oz - xyzmin) / voxel_size;
+        bool inside = (px >= 0) & (py >= 0) & (pz >= 0) & (px <= nf) & (py <= nf) & (pz <= nf);
+        if (!inside) {
+            float tmin;
+            if (!box_entry(px, py, pz, dx, dy, dz, nf, px, py, pz, tmin)) return;
+        }
+        px /= 64.0f;
+        py /= 64.0f;
+        pz /= 64.0f;
+        float qx = px, qy = py, qz = pz;
+        int m = n / 64;
+        int64_t off = 0, total = 0;
+        for (int k = n / 64; k < n; k *= 4) total += (int64_t)k * k * k;
+        const int bound = max_dda_steps(nf) + 16;
+        for (int it = 0; it < bound && m < n; ++it) {
+            qx = px;
+            qy = py;
+            qz = pz;
+            dda_step(qx, qy, qz, dx, dy, dz, sx, sy, sz, px, py, pz);
+            if (outside(px, py, pz, (float)m)) return;
+            int vx, vy, vz;
+            segment_voxel1(qx, qy, qz, px, py, pz, vx, vy, vz);
+            int64_t m3 = (int64_t)m * m * m;
+            int64_t ci = ((int64_t)vz * m + vy) * m + vx;
+            int64_t gi = off + ci;
+            bool occ = (ci >= 0 && ci < m3 && gi < total) ? octrees[gi] != 0 : false;
+            if (occ) {
+                off += m3;
+                qx *= 4.0f;
+                qy *= 4.0f;
+                qz *= 4.0f;
+                px = qx;
+                py = qy;
+                pz = qz;
+                m *= 4;
+            }
+        }
+        if (m < n) return;
+        lx = qx;
+        ly = qy;
+        lz = qz;
+        alive = true;
+    }
+
+    // Advance to the next occupied voxel; on success (p, q) is its (entry, exit).  OPEN selects
+    // the real-time exit test (leave at >= N, ray_march.cu:203-204) instead of > N.
+    template <bool OPEN = false>
+    __device__ __forceinline__ bool next(const uint8_t* __restrict__ mask, int n, float& px,
+                                         float& py, float& pz, float& qx, float& qy, float& qz) {
+        const float nf = (float)n;
+        const int64_t ntot = (int64_t)n * n * n;
+        while (alive && steps < max_steps) {
+            ++steps;
+            float nx, ny, nz;
+            dda_step(lx, ly, lz, dx, dy, dz, sx, sy, sz, nx, ny, nz);
+            if (OPEN ? outside_open(nx, ny, nz, nf) : outside(nx, ny, nz, nf)) break;
+            int vx, vy, vz;
+            segment_voxel1(lx, ly, lz, nx, ny, nz, vx, vy, vz);
+            int64_t ci = ((int64_t)vz * n + vy) * n + vx;
+            bool occ = (ci >= 0 && ci < ntot) ? mask[ci] != 0 : false;
+            px = lx; py = ly; pz = lz;
+            lx = nx; ly = ny; lz = nz;
+            if (occ) {
+                qx = nx; qy = ny; qz = nz;
+                return true;
+            }
+        }
+        alive = false;
+        return false;
+    }
+};
+
+// Corner weights of the segment mean of the trilinear field (integrator.py:92-111 arranged as
+// mlp_eval.cu:83-101) and the 8 corner offsets into a [(n+1)^3][C] grid.
+template <int C>
+__device__ __forceinline__ void segment_corners(float px, float py, float pz, float qx, float qy,
+                                                float qz, int n, float (&w)[8], int64_t (&off)[8]) {
+    int cx, cy, cz;
+    {
+#pragma clang fp contract(off)
+        int ax = (int)(px + 1e-4f), bx = (int)(qx + 1e-4f);
+        int ay = (int)(py + 1e-4f), by = (int)(qy + 1e-4f);
+        int az = (int)(pz + 1e-4f), bz = (int)(qz + 1e-4f);
+        cx = ax < bx ? ax : bx;
+        cy = ay < by ? ay : by;
+        cz = az < bz ? az : bz;
+    }
+    float x0 = px - cx, x1 = qx - cx, y0 = py - cy, y1 = qy - cy, z0 = pz - cz, z1 = qz - cz;
+    float z01 = z0 + z1, y01 = y0 + y1, x01 = x0 + x1;
+    w[7] = (2.0f * x0 * y0 * z0 + 2.0f * x1 * y1 * z1 + x01 * y01 * z01) * (1.0f / 12.0f);
+    float w2 = (y01 * z01 + y0 * z0 + y1 * z1) * (1.0f / 6.0f);
+    w[5] = (x01 * z01 + x0 * z0 + x1 * z1) * (1.0f / 6.0f) - w[7];
+    float w4 = (x01 * y01 + x0 * y0 + x1 * y1) * (1.0f / 6.0f);
+    x01 *= 0.5f;
+    y01 *= 0.5f;
+    z01 *= 0.5f;
+    w[4] = z01 - w[5] - w2;
+    w[0] = 1.0f - w[4] - y01 - x01 + w4;
+    w[6] = w2 - w[7];
+    w[2] = y01 - w[6] - w4;
+    w[1] = x01 - w[5] - w4;
+    w[3] = w4 - w[7];
+    const int cx1 = min(cx + 1, n), cy1 = min(cy + 1, n), cz1 = min(cz + 1, n);
+    const int64_t sz = (int64_t)(n + 1) * (n + 1) * C, sy = (int64_t)(n + 1) * C;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        int gx = (j & 1) ? cx1 : cx;
+        int gy = ((j >> 1) & 1) ? cy1 : cy;
+        int gz = (j >> 2) ? cz1 : cz;
+        off[j] = gz * sz + gy * sy + (int64_t)gx * C;
+    }
+}
+
+template <int C>
+__device__ __forceinline__ void gather_integral(const float* __restrict__ voxels, const float (&w)[8],
+                                                const int64_t (&off)[8], float (&f)[C]) {
+#pragma unroll
+    for (int i = 0; i < C; ++i) f[i] = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const float4* src = reinterpret_cast<const float4*>(voxels + off[j]);
+#pragma unroll
+        for (int i = 0; i < C / 4; ++i) {
+            float4 q = src[i];
+            f[4 * i + 0] += w[j] * q.x;
+            f[4 * i + 1] += w[j] * q.y;
+            f[4 * i + 2] += w[j] * q.z;
+            f[4 * i + 3] += w[j] * q.w;
+        }
+    }
+}
+
+// Pinhole camera ray of pixel (x, y) (FP32): utils/dataset/ray_utils.py:5-56 conventions.
+__device__ __forceinline__ void camera_ray(const CameraDesc& cam, int x, int y, float& ox, float& oy,
+                                           float& oz, float& dx, float& dy, float& dz) {
+#pragma clang fp contract(off)
+    float cx = ((float)x + 0.5f - 0.5f * (float)cam.width) / cam.focal;
+    float cy = -(((float)y + 0.5f - 0.5f * (float)cam.height) / cam.focal);
+    float cz = -1.0f;
+    const float* m = cam.c2w;  // row-major 3x4
+    float wx = m[0] * cx + m[1] * cy + m[2] * cz;
+    float wy = m[4] * cx + m[5] * cy + m[6] * cz;
+    float wz = m[8] * cx + m[9] * cy + m[10] * cz;
+    float nrm = sqrtf(wx * wx + wy * wy + wz * wz);
+    dx = wx / nrm;
+    dy = wy / nrm;
+    dz = wz / nrm;
+    ox = m[3];
+    oy = m[7];
+    oz = m[11];
+}
+
+// Ray r of a camera render in tile order: 8x8 pixel tiles, row-major over the image.
+__device__ __forceinline__ bool camera_pixel(const CameraDesc& cam, int64_t r, int& x, int& y) {
+    int tiles_x = (cam.width + 7) >> 3;
+    int64_t tile = r >> 6;
+    int in = (int)(r & 63);
+    x = (int)(tile % tiles_x) * 8 + (in & 7);
+    y = (int)(tile / tiles_x) * 8 + (in >> 3);
+    return x < cam.width && y < cam.height;
+}
+
+struct Kahan {
+    float s = 0.0f, c = 0.0f;
+    __device__ __forceinline__ void add(float x) {
+#pragma clang fp contract(off)
+        float y = x - c;
+        float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+};
+
+__device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0; }
+
+}  // namespace diver

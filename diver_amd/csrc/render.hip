@@ -16,40 +16,9 @@
 #include "decoder.h"
 #include "kernels.h"
 #include "mfma.h"
+#include "raystate.h"
 
 namespace diver {
-
-__device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0; }
-
-// Pinhole camera ray of pixel (x, y) (FP32; same arithmetic as launch_camera_rays' kernel).
-__device__ __forceinline__ void camera_ray(const CameraDesc& cam, int x, int y, float& ox, float& oy,
-                                           float& oz, float& dx, float& dy, float& dz) {
-#pragma clang fp contract(off)
-    float cx = ((float)x + 0.5f - 0.5f * (float)cam.width) / cam.focal;
-    float cy = -(((float)y + 0.5f - 0.5f * (float)cam.height) / cam.focal);
-    float cz = -1.0f;
-    const float* m = cam.c2w;  // row-major 3x4
-    float wx = m[0] * cx + m[1] * cy + m[2] * cz;
-    float wy = m[4] * cx + m[5] * cy + m[6] * cz;
-    float wz = m[8] * cx + m[9] * cy + m[10] * cz;
-    float nrm = sqrtf(wx * wx + wy * wy + wz * wz);
-    dx = wx / nrm;
-    dy = wy / nrm;
-    dz = wz / nrm;
-    ox = m[3];
-    oy = m[7];
-    oz = m[11];
-}
-
-// Ray r of a camera render in tile order: 8x8 tiles, row-major over the image.
-__device__ __forceinline__ bool camera_pixel(const CameraDesc& cam, int64_t r, int& x, int& y) {
-    int tiles_x = (cam.width + 7) >> 3;
-    int64_t tile = r >> 6;
-    int in = (int)(r & 63);
-    x = (int)(tile % tiles_x) * 8 + (in & 7);
-    y = (int)(tile / tiles_x) * 8 + (in >> 3);
-    return x < cam.width && y < cam.height;
-}
 
 __global__ void camera_rays_kernel(CameraDesc cam, int64_t n, float* __restrict__ o,
                                    float* __restrict__ d) {
@@ -62,18 +31,7 @@ __global__ void camera_rays_kernel(CameraDesc cam, int64_t n, float* __restrict_
     d[3 * p] = dx; d[3 * p + 1] = dy; d[3 * p + 2] = dz;
 }
 
-struct Kahan {
-    float s = 0.0f, c = 0.0f;
-    __device__ __forceinline__ void add(float x) {
-#pragma clang fp contract(off)
-        float y = x - c;
-        float t = s + y;
-        c = (t - s) - y;
-        s = t;
-    }
-};
-
-template <class D, int PREC>
+template <class D, int PREC, bool RT>
 __global__ void __launch_bounds__(256) render_kernel(RenderArgs a) {
     constexpr int C = D::C, H = D::H, Z = D::Z;
     const int lane = threadIdx.x & 63;
@@ -96,25 +54,11 @@ __global__ void __launch_bounds__(256) render_kernel(RenderArgs a) {
         ox = a.o[3 * rr]; oy = a.o[3 * rr + 1]; oz = a.o[3 * rr + 2];
         dx = a.d[3 * rr]; dy = a.d[3 * rr + 1]; dz = a.d[3 * rr + 2];
     }
-    const float n = (float)a.n;
-    float lx, ly, lz;
-    bool alive;
-    {
-#pragma clang fp contract(off)
-        float gx = (ox - a.xyzmin) / a.voxel_size;
-        float gy = (oy - a.xyzmin) / a.voxel_size;
-        float gz = (oz - a.xyzmin) / a.voxel_size;
-        lx = gx; ly = gy; lz = gz;
-        bool inside = (gx >= 0) & (gy >= 0) & (gz >= 0) & (gx <= n) & (gy <= n) & (gz <= n);
-        alive = valid;
-        if (valid && !inside) {
-            float tmin;
-            alive = box_entry(gx, gy, gz, dx, dy, dz, n, lx, ly, lz, tmin);
-        }
-    }
-    const float sx = (float)sgnf(dx), sy = (float)sgnf(dy), sz = (float)sgnf(dz);
-    const int max_steps = max_dda_steps(n);
-    int steps = 0;
+    RayWalk ray;
+    if constexpr (RT)
+        ray.init_rt(ox, oy, oz, dx, dy, dz, a.xyzmin, a.voxel_size, a.n, a.octrees, valid);
+    else
+        ray.init(ox, oy, oz, dx, dy, dz, a.xyzmin, a.voxel_size, (float)a.n, valid);
 
     // per-ray view term of the colour head (FP32)
     float hv[H];
@@ -144,84 +88,21 @@ __global__ void __launch_bounds__(256) render_kernel(RenderArgs a) {
     Kahan S;
     float T = 1.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f;
     int used = 0;
-    const int64_t sz_ = (int64_t)(a.n + 1) * (a.n + 1) * C, sy_ = (int64_t)(a.n + 1) * C;
-    const int64_t ntot = (int64_t)a.n * a.n * a.n;
-
     while (true) {
         // ---- traverse to the next occupied voxel ---------------------------------------------
-        bool has = false;
         float px = 0.f, py = 0.f, pz = 0.f, qx = 0.f, qy = 0.f, qz = 0.f;
-        if (alive) {
-            while (steps < max_steps) {
-                ++steps;
-                float nx, ny, nz;
-                dda_step(lx, ly, lz, dx, dy, dz, sx, sy, sz, nx, ny, nz);
-                if (outside(nx, ny, nz, n)) { steps = max_steps; break; }
-                int vx, vy, vz;
-                segment_voxel1(lx, ly, lz, nx, ny, nz, vx, vy, vz);
-                int64_t ci = ((int64_t)vz * a.n + vy) * a.n + vx;
-                bool occ = (ci >= 0 && ci < ntot) ? a.mask[ci] != 0 : false;
-                if (occ) {
-                    px = lx; py = ly; pz = lz;
-                    qx = nx; qy = ny; qz = nz;
-                    lx = nx; ly = ny; lz = nz;
-                    has = true;
-                    break;
-                }
-                lx = nx; ly = ny; lz = nz;
-            }
-            if (!has) alive = false;
-        }
+        bool has = ray.next<RT>(a.mask, a.n, px, py, pz, qx, qy, qz);
         if (!wave_any(has)) break;
 
         // ---- segment integral (FP32) ----------------------------------------------------------
         float f[C];
-#pragma unroll
-        for (int i = 0; i < C; ++i) f[i] = 0.0f;
         float sigma = 0.0f;
         float z[Z];
         if (has) {
-            int cx, cy, cz;
-            {
-#pragma clang fp contract(off)
-                int ax = (int)(px + 1e-4f), bx = (int)(qx + 1e-4f);
-                int ay = (int)(py + 1e-4f), by = (int)(qy + 1e-4f);
-                int az = (int)(pz + 1e-4f), bz = (int)(qz + 1e-4f);
-                cx = ax < bx ? ax : bx;
-                cy = ay < by ? ay : by;
-                cz = az < bz ? az : bz;
-            }
-            float x0 = px - cx, x1 = qx - cx, y0 = py - cy, y1 = qy - cy, z0 = pz - cz, z1 = qz - cz;
-            float z01 = z0 + z1, y01 = y0 + y1, x01 = x0 + x1;
             float w[8];
-            w[7] = (2.0f * x0 * y0 * z0 + 2.0f * x1 * y1 * z1 + x01 * y01 * z01) * (1.0f / 12.0f);
-            float w2 = (y01 * z01 + y0 * z0 + y1 * z1) * (1.0f / 6.0f);
-            w[5] = (x01 * z01 + x0 * z0 + x1 * z1) * (1.0f / 6.0f) - w[7];
-            float w4 = (x01 * y01 + x0 * y0 + x1 * y1) * (1.0f / 6.0f);
-            x01 *= 0.5f; y01 *= 0.5f; z01 *= 0.5f;
-            w[4] = z01 - w[5] - w2;
-            w[0] = 1.0f - w[4] - y01 - x01 + w4;
-            w[6] = w2 - w[7];
-            w[2] = y01 - w[6] - w4;
-            w[1] = x01 - w[5] - w4;
-            w[3] = w4 - w[7];
-            const int cx1 = min(cx + 1, a.n), cy1 = min(cy + 1, a.n), cz1 = min(cz + 1, a.n);
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                int gx = (j & 1) ? cx1 : cx;
-                int gy = ((j >> 1) & 1) ? cy1 : cy;
-                int gz = (j >> 2) ? cz1 : cz;
-                const float4* src =
-                    reinterpret_cast<const float4*>(a.voxels + gz * sz_ + gy * sy_ + (int64_t)gx * C);
-#pragma unroll
-                for (int i = 0; i < C / 4; ++i) {
-                    float4 q = src[i];
-                    f[4 * i + 0] += w[j] * q.x;
-                    f[4 * i + 1] += w[j] * q.y;
-                    f[4 * i + 2] += w[j] * q.z;
-                    f[4 * i + 3] += w[j] * q.w;
-                }
-            }
+            int64_t off[8];
+            segment_corners<C>(px, py, pz, qx, qy, qz, a.n, w, off);
+            gather_integral<C>(a.voxels, w, off, f);
             sigma = density_act<D>(pre_and_sigma<D>(a.W, f, z));
         } else {
 #pragma unroll
@@ -300,13 +181,14 @@ __global__ void __launch_bounds__(256) render_kernel(RenderArgs a) {
         if (has) {
             used += 1;
             float alpha = -expm1f(-sigma);
+            if (alpha < a.alpha_skip) continue;
             float wgt = T * alpha;
             cr += wgt * rgb[0];
             cg += wgt * rgb[1];
             cb += wgt * rgb[2];
             S.add(sigma);
             T = __expf(-S.s);
-            if (T < a.t_stop) alive = false;
+            if (T < a.t_stop) ray.alive = false;
         }
     }
     if (valid) {
@@ -316,31 +198,37 @@ __global__ void __launch_bounds__(256) render_kernel(RenderArgs a) {
     }
 }
 
-template <class D>
+template <class D, bool RT>
 static hipError_t launch_render_t(const RenderArgs& a, int prec, hipStream_t s) {
     int64_t waves = (a.n_rays + 63) / 64;
     dim3 grid((unsigned)((waves + 3) / 4)), block(256);
     if (prec == kPrecFP32) {
-        hipLaunchKernelGGL((render_kernel<D, kPrecFP32>), grid, block, 0, s, a);
+        hipLaunchKernelGGL((render_kernel<D, kPrecFP32, RT>), grid, block, 0, s, a);
     } else if constexpr (D::NPRE == 0) {
         if (prec == kPrecBF16)
-            hipLaunchKernelGGL((render_kernel<D, kPrecBF16>), grid, block, 0, s, a);
+            hipLaunchKernelGGL((render_kernel<D, kPrecBF16, RT>), grid, block, 0, s, a);
         else
-            hipLaunchKernelGGL((render_kernel<D, kPrecFP16>), grid, block, 0, s, a);
+            hipLaunchKernelGGL((render_kernel<D, kPrecFP16, RT>), grid, block, 0, s, a);
     } else {
         return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
+template <class D>
+static hipError_t launch_render_d(const RenderArgs& a, int prec, hipStream_t s) {
+    return a.traversal == 1 ? launch_render_t<D, true>(a, prec, s)
+                            : launch_render_t<D, false>(a, prec, s);
+}
+
 hipError_t launch_render(int variant, const RenderArgs& a, int prec, hipStream_t s) {
     if (a.n_rays == 0) return hipSuccess;
     switch (variant) {
-        case 0: return launch_render_t<DecB32>(a, prec, s);
-        case 1: return launch_render_t<DecB64>(a, prec, s);
-        case 2: return launch_render_t<DecB64C64>(a, prec, s);
-        case 3: return launch_render_t<DecR32>(a, prec, s);
-        case 4: return launch_render_t<DecR64>(a, prec, s);
+        case 0: return launch_render_d<DecB32>(a, prec, s);
+        case 1: return launch_render_d<DecB64>(a, prec, s);
+        case 2: return launch_render_d<DecB64C64>(a, prec, s);
+        case 3: return launch_render_d<DecR32>(a, prec, s);
+        case 4: return launch_render_d<DecR64>(a, prec, s);
         default: return hipErrorInvalidValue;
     }
 }

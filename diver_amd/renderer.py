@@ -8,12 +8,13 @@ import numpy as np
 import torch
 
 from . import calls
-from .decoder import CanonicalDecoder, Decoder
+from .decoder import Decoder
+from .utils.ray_march import build_octrees
 
 
 class SceneGrid:
-    """A scene resident in device memory: dense corner features [(N+1)^3, C] fp32 and the
-    [N^3] occupancy mask, over [xyzmin, xyzmin + N*voxel_size]^3."""
+    """A scene resident in device memory: dense corner features [(N+1)^3, C] fp32, the [N^3]
+    occupancy mask and its 64/16/4 occupancy pyramid, over [xyzmin, xyzmin + N*voxel_size]^3."""
 
     def __init__(self, voxels: torch.Tensor, mask: torch.Tensor, xyzmin: float, voxel_size: float):
         if voxels.dim() != 4 or mask.dim() != 3:
@@ -27,6 +28,8 @@ class SceneGrid:
         self.channels = voxels.shape[-1]
         self.xyzmin = float(xyzmin)
         self.voxel_size = float(voxel_size)
+        self.octrees = (build_octrees(self.mask.bool()).to(torch.uint8).contiguous()
+                        if n % 64 == 0 else None)
 
     @classmethod
     def from_synthetic(cls, scene, device) -> 'SceneGrid':
@@ -36,14 +39,22 @@ class SceneGrid:
 
     @property
     def nbytes(self) -> int:
-        return self.voxels.numel() * 4 + self.mask.numel()
+        extra = self.octrees.numel() if self.octrees is not None else 0
+        return self.voxels.numel() * 4 + self.mask.numel() + extra
+
+
+def _walk(grid: SceneGrid, traversal: str) -> str:
+    if traversal == 'auto':
+        return 'realtime' if grid.octrees is not None else 'offline'
+    return traversal
 
 
 def render_rays(grid: SceneGrid, decoder: Decoder, o: torch.Tensor, d: torch.Tensor,
-                precision: str = 'fp32', t_stop: float = 1e-4, white_back: bool = True):
+                precision: str = 'fp32', t_stop: float = 1e-4, white_back: bool = True,
+                traversal: str = 'auto', alpha_skip: float = 0.0):
     """Returns (rgb [R,3], transmittance [R], intervals composited [R])."""
-    rgba, count = calls.render_rays(grid.voxels, grid.mask, grid.xyzmin, grid.voxel_size,
-                                    decoder, o, d, precision, t_stop)
+    opts = calls.render_opts(precision, _walk(grid, traversal), t_stop, alpha_skip)
+    rgba, count = calls.render_rays(grid, decoder, o, d, opts)
     rgb = rgba[:, :3] + rgba[:, 3:] if white_back else rgba[:, :3]
     return rgb, rgba[:, 3], count
 
@@ -54,12 +65,13 @@ def focal_for(width: int, fov: float) -> float:
 
 def render_camera(grid: SceneGrid, decoder: Decoder, c2w, width: int, height: int,
                   fov: float = 0.69, precision: str = 'fp32', t_stop: float = 1e-4,
-                  white_back: bool = True, with_count: bool = False):
+                  white_back: bool = True, with_count: bool = False, traversal: str = 'auto',
+                  alpha_skip: float = 0.0):
     """One pinhole view: returns (rgb [H, W, 3], transmittance [H, W], count or None)."""
     c2w = np.asarray(c2w, np.float32).reshape(3, 4)
-    rgba, count = calls.render_camera(grid.voxels, grid.mask, grid.xyzmin, grid.voxel_size,
-                                      decoder, c2w, width, height, focal_for(width, fov),
-                                      precision, t_stop, with_count)
+    opts = calls.render_opts(precision, _walk(grid, traversal), t_stop, alpha_skip)
+    rgba, count = calls.render_camera(grid, decoder, c2w, width, height, focal_for(width, fov),
+                                      opts, with_count)
     rgb = rgba[..., :3] + rgba[..., 3:] if white_back else rgba[..., :3]
     return rgb, rgba[..., 3], count
 

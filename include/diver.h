@@ -146,8 +146,10 @@ int diver_mlp_eval(const diver_decoder* dec, int64_t batch, const float* coord,
 /* ---- fused renderer --------------------------------------------------------------------------- */
 
 typedef struct diver_grid {
-    const float* voxels;  /* [(n+1)^3][channels] */
-    const uint8_t* mask;  /* [n^3] */
+    const float* voxels;    /* [(n+1)^3][channels] */
+    const uint8_t* mask;    /* [n^3] */
+    const uint8_t* octrees; /* occupancy pyramid for the real-time walk (block sizes 64, 16, 4,
+                               coarse first, eval_rt.py:236-239); NULL for the offline walk */
     int n;
     int channels;
     float xyzmin;
@@ -156,19 +158,33 @@ typedef struct diver_grid {
 
 enum diver_precision { DIVER_FP32 = 0, DIVER_BF16 = 1, DIVER_FP16 = 2 };
 
-/* Traversal + integral + decoder + compositing of explicit rays in one launch (the fused path
- * of DIVeR.forward + DIVeR.render, model/diver.py:141-207, with front-to-back early
- * termination at transmittance < t_stop).  rgba [n_rays][4] = (accumulated rgb, T); add T for
- * a white background.  count [n_rays] (optional) = intervals composited. */
-int diver_render_rays(const diver_grid* grid, const diver_decoder* dec, int precision,
-                      int64_t n_rays, const float* o, const float* d, float t_stop, float* rgba,
-                      int32_t* count, diver_stream_t stream);
+enum diver_traversal {
+    DIVER_WALK_OFFLINE = 0,  /* masked_intersect's walk from the box entry, exit at > N       */
+    DIVER_WALK_REALTIME = 1  /* aabb_intersect's octree descent, then ray_march's walk (>= N) */
+};
+
+typedef struct diver_render_opts {
+    int precision;     /* diver_precision: decoder arithmetic (traversal/integral stay FP32) */
+    int traversal;     /* diver_traversal; n must be a multiple of 64 for the real-time walk  */
+    float t_stop;      /* stop a ray once transmittance < t_stop (0 = never)                 */
+    float alpha_skip;  /* skip intervals with alpha < alpha_skip, as mlp_eval.cu:153 (0 = off) */
+} diver_render_opts;
+
+/* Traversal + integral + decoder + compositing of explicit rays in one launch: the fused form of
+ * eval_rt.py's aabb_intersect / ray_march / mlp_eval loop (or of DIVeR.forward + render,
+ * model/diver.py:141-207, with the offline walk).  Transmittance is exp(-S) with S a compensated
+ * FP32 sum of densities.  rgba [n_rays][4] = (accumulated rgb, T); add T for a white background.
+ * count [n_rays] (optional) = intervals composited. */
+int diver_render_rays(const diver_grid* grid, const diver_decoder* dec,
+                      const diver_render_opts* opts, int64_t n_rays, const float* o,
+                      const float* d, float* rgba, int32_t* count, diver_stream_t stream);
 
 /* Same for a pinhole camera (utils/dataset/ray_utils.py:5-56 conventions: pixel centres +0.5,
  * -z forward, normalised directions); c2w_host is row-major 3x4.  rgba/count are [height][width]. */
-int diver_render_camera(const diver_grid* grid, const diver_decoder* dec, int precision,
-                        const float* c2w_host, int width, int height, float focal, float t_stop,
-                        float* rgba, int32_t* count, diver_stream_t stream);
+int diver_render_camera(const diver_grid* grid, const diver_decoder* dec,
+                        const diver_render_opts* opts, const float* c2w_host, int width,
+                        int height, float focal, float* rgba, int32_t* count,
+                        diver_stream_t stream);
 
 /* The FP32 rays diver_render_camera traces: o, d [height*width][3]. */
 int diver_camera_rays(const float* c2w_host, int width, int height, float focal, float* o,

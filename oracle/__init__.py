@@ -44,6 +44,8 @@ def lib():
                                               u8p, f32p, i32p, f32p]
         L.oracle_aabb_intersect.argtypes = [i64, f32p, f32p, f32p, u8p, f32p, u8p, ctypes.c_int]
         L.oracle_ray_march.argtypes = [i64, f32p, f32p, u8p, u8p, ctypes.c_int, i64]
+        L.oracle_rt_walk.argtypes = [i64, ctypes.c_int, fl, fl, ctypes.c_int, f32p, f32p, u8p,
+                                     u8p, f32p, i32p]
         _LIB = L
     return _LIB
 
@@ -108,6 +110,22 @@ def ray_march(coords, v, mask, finish, voxel_num, stride=REF_RAY_MARCH_STRIDE):
                            _p(np.ascontiguousarray(mask, np.uint8), ctypes.c_uint8),
                            _p(np.ascontiguousarray(finish, np.uint8), ctypes.c_uint8),
                            int(voxel_num), int(stride))
+
+
+def rt_walk(o, v, xyzmin, voxel_size, voxel_num, octrees, mask, max_n=None):
+    """Whole real-time walk (aabb_intersect + ray_march rounds) per ray, own origin per ray.
+    Returns (isect [B,max_n,6], num [B]) in masked_intersect's layout."""
+    o, v = _f32(o), _f32(v)
+    B = o.shape[0]
+    max_n = 3 * voxel_num if max_n is None else max_n
+    isect = np.zeros((B, max_n, 6), np.float32)
+    num = np.zeros(B, np.int32)
+    lib().oracle_rt_walk(B, max_n, np.float32(xyzmin), np.float32(voxel_size), int(voxel_num),
+                         _p(o, ctypes.c_float), _p(v, ctypes.c_float),
+                         _p(np.ascontiguousarray(octrees, np.uint8), ctypes.c_uint8),
+                         _p(np.ascontiguousarray(mask, np.uint8), ctypes.c_uint8),
+                         _p(isect, ctypes.c_float), _p(num, ctypes.c_int32))
+    return isect, num
 
 
 def octree_levels(mask: np.ndarray) -> np.ndarray:

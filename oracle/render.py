@@ -1,21 +1,29 @@
 """CPU ORACLE -- test infrastructure only.
 
 FP64 re-evaluation of a full render: the FP32 traversal restatement (traverse.c) gives the
-occupied intervals; the integral, decoder and front-to-back compositing then run in float64
-(fp64.py).  Density activations: 0 = relu (model/diver.py:195), 1 = softplus (benchmark decoder).
+occupied intervals -- masked_intersect's walk ('offline') or the real-time walk of
+aabb_intersect + ray_march ('realtime') -- and the integral, decoder and front-to-back compositing
+then run in float64 (fp64.py).  Density activations: 0 = relu (model/diver.py:195),
+1 = softplus (benchmark decoder).
 """
 from __future__ import annotations
 
 import numpy as np
 
-from . import fp64, masked_intersect
+from . import fp64, masked_intersect, octree_levels, rt_walk
 
 
-def intervals(o, d, mask, n, xyzmin, voxel_size, max_n=None):
+def intervals(o, d, mask, n, xyzmin, voxel_size, max_n=None, walk='offline'):
     """Rows (ray index, p_in, p_out) of every occupied interval, in ray then depth order."""
     max_n = 3 * n if max_n is None else max_n
-    isect, num, _ = masked_intersect(o, d, np.float32(xyzmin), np.float32(xyzmin + n * voxel_size),
-                                     n, np.float32(voxel_size), mask, 1.0, max_n=max_n)
+    if walk == 'offline':
+        isect, num, _ = masked_intersect(o, d, np.float32(xyzmin),
+                                         np.float32(xyzmin + n * voxel_size), n,
+                                         np.float32(voxel_size), mask, 1.0, max_n=max_n)
+    elif walk == 'realtime':
+        isect, num = rt_walk(o, d, xyzmin, voxel_size, n, octree_levels(mask), mask, max_n)
+    else:
+        raise ValueError(walk)
     if num.max(initial=0) > max_n:
         raise ValueError('max_n too small for these rays')
     ray = np.repeat(np.arange(o.shape[0]), num)
@@ -25,9 +33,9 @@ def intervals(o, d, mask, n, xyzmin, voxel_size, max_n=None):
 
 
 def render(voxels, mask, n, xyzmin, voxel_size, decode, density_act, o, d, t_stop=1e-4,
-           alpha_skip=0.0, max_n=None):
+           alpha_skip=0.0, max_n=None, walk='offline'):
     """Returns dict(rgb [R,3] without background, T [R], used [R], num [R])."""
-    ray, pin, pout, num = intervals(o, d, mask, n, xyzmin, voxel_size, max_n)
+    ray, pin, pout, num = intervals(o, d, mask, n, xyzmin, voxel_size, max_n, walk)
     R = o.shape[0]
     if ray.size == 0:
         return dict(rgb=np.zeros((R, 3)), T=np.ones(R), used=np.zeros(R, np.int64), num=num)

@@ -294,6 +294,85 @@ void oracle_aabb_intersect(int64_t batch, float *coords, const float *v, const f
     }
 }
 
+/* Whole real-time walk of each ray: aabb_intersect's start point (its own origin per ray, grid
+ * units (o - xyzmin)/voxel_size as eval_rt.py:110 computes the shared centre) followed by
+ * ray_march rounds until the ray leaves at >= N.  Rounds do not change the recurrence (a round
+ * resumes from the exit of the previous round's last interval), so this emits exactly the
+ * intervals that aabb_intersect + repeated ray_march produce.  Output as masked_intersect. */
+void oracle_rt_walk(int64_t batch, int max_n, float xyzmin, float voxel_size, int voxel_num,
+                    const float *o, const float *v, const uint8_t *octrees, const uint8_t *mask,
+                    float *isect, int32_t *num) {
+    float n = (float)voxel_num;
+    int64_t oct_total = 0;
+    for (int m = voxel_num / 64; m < voxel_num; m *= 4) oct_total += (int64_t)m * m * m;
+    int64_t total = (int64_t)voxel_num * voxel_num * voxel_num;
+    int max_steps = max_steps_for(n);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int64_t b = 0; b < batch; ++b) {
+        num[b] = 0;
+        float px = (o[3 * b] - xyzmin) / voxel_size;
+        float py = (o[3 * b + 1] - xyzmin) / voxel_size;
+        float pz = (o[3 * b + 2] - xyzmin) / voxel_size;
+        float dx = v[3 * b], dy = v[3 * b + 1], dz = v[3 * b + 2];
+        int inside = (px >= 0) & (py >= 0) & (pz >= 0) & (px <= n) & (py <= n) & (pz <= n);
+        if (!inside) {
+            float tmin;
+            if (!box_entry(px, py, pz, dx, dy, dz, n, &px, &py, &pz, &tmin)) continue;
+        }
+        px /= 64.0f; py /= 64.0f; pz /= 64.0f;
+        float lx = px, ly = py, lz = pz;
+        float sx = (float)sgn(dx), sy = (float)sgn(dy), sz = (float)sgn(dz);
+        int m = voxel_num / 64;
+        int64_t off = 0;
+        int dead = 0;
+        for (int it = 0; it < max_steps + 16 && m < voxel_num; ++it) {
+            lx = px; ly = py; lz = pz;
+            dda_step(lx, ly, lz, dx, dy, dz, sx, sy, sz, &px, &py, &pz);
+            float ms = (float)m;
+            if ((px < 0) | (py < 0) | (pz < 0) | (px > ms) | (py > ms) | (pz > ms)) {
+                dead = 1;
+                break;
+            }
+            float cx = fminf(lx + 1e-4f, px + 1e-4f);
+            float cy = fminf(ly + 1e-4f, py + 1e-4f);
+            float cz = fminf(lz + 1e-4f, pz + 1e-4f);
+            int64_t m3 = (int64_t)m * m * m;
+            int64_t ci = (int64_t)(int)cz * m * m + (int64_t)(int)cy * m + (int)cx;
+            int64_t gi = off + ci;
+            int occ = (ci >= 0 && ci < m3 && gi < oct_total) ? octrees[gi] != 0 : 0;
+            if (occ) {
+                off += m3;
+                lx *= 4.0f; ly *= 4.0f; lz *= 4.0f;
+                px = lx; py = ly; pz = lz;
+                m *= 4;
+            }
+        }
+        if (dead || m < voxel_num) continue;
+        int32_t count = 0;
+        for (int it = 0; it < max_steps; ++it) {
+            float qx, qy, qz;
+            dda_step(lx, ly, lz, dx, dy, dz, sx, sy, sz, &qx, &qy, &qz);
+            if ((qx < 0) | (qy < 0) | (qz < 0) | (qx >= n) | (qy >= n) | (qz >= n)) break;
+            float cx = fminf(lx + 1e-4f, qx + 1e-4f);
+            float cy = fminf(ly + 1e-4f, qy + 1e-4f);
+            float cz = fminf(lz + 1e-4f, qz + 1e-4f);
+            int64_t ci = (int64_t)(int)cz * voxel_num * voxel_num + (int64_t)(int)cy * voxel_num +
+                         (int)cx;
+            int occ = (ci >= 0 && ci < total) ? mask[ci] != 0 : 0;
+            if (occ) {
+                if (count < max_n) {
+                    float *r = isect + ((int64_t)b * max_n + count) * 6;
+                    r[0] = lx; r[1] = ly; r[2] = lz;
+                    r[3] = qx; r[4] = qy; r[5] = qz;
+                }
+                count += 1;
+            }
+            lx = qx; ly = qy; lz = qz;
+        }
+        num[b] = count;
+    }
+}
+
 /* One round of the real-time fine march.  `stride` reproduces the reference's grid-stride
  * assignment (NUM_BLOCKS*NUM_THREADS = 262144 rays per thread sweep): a thread that meets a
  * resumable ray with a negative resume point leaves its loop, so the later rays of that thread

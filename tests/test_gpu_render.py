@@ -28,7 +28,7 @@ def psnr(a, b):
     return float('inf') if mse == 0 else -10 * np.log10(mse)
 
 
-def _oracle(sc, dec_kind, layers, o, d, t_stop=1e-4):
+def _oracle(sc, dec_kind, layers, o, d, t_stop=1e-4, walk='realtime'):
     vox = sc.dense_voxels()
     if dec_kind == 'baseline':
         decode = lambda F, dd: fp64.baseline_decoder(layers, F, dd)
@@ -36,26 +36,29 @@ def _oracle(sc, dec_kind, layers, o, d, t_stop=1e-4):
     else:
         decode = lambda F, dd: fp64.mlp_chain(layers, F, dd)
         act = 0
-    return oracle_render(vox, sc.mask, sc.n, sc.xyzmin, sc.voxel_size, decode, act, o, d, t_stop)
+    return oracle_render(vox, sc.mask, sc.n, sc.xyzmin, sc.voxel_size, decode, act, o, d, t_stop,
+                         walk=walk)
 
 
 def _count_ok(got, ref):
     return np.mean(got == ref)
 
 
+@pytest.mark.parametrize('walk', ['realtime', 'offline'])
 @pytest.mark.parametrize('hidden', [32, 64])
 @pytest.mark.parametrize('precision', ['fp32', 'bf16', 'fp16'])
-def test_render_camera_baseline_decoder(scene64, cameras, dev, hidden, precision):
+def test_render_camera_baseline_decoder(scene64, cameras, dev, hidden, precision, walk):
     sc = scene64
     layers = S.baseline_decoder_layers(32, hidden)
     dec = Decoder(from_baseline(layers, 32))
     grid = SceneGrid.from_synthetic(sc, dev)
     W = H = 64
     for cam in (cameras[0], cameras[11]):
-        rgb, T, cnt = render_camera(grid, dec, cam, W, H, precision=precision, with_count=True)
+        rgb, T, cnt = render_camera(grid, dec, cam, W, H, precision=precision, with_count=True,
+                                    traversal=walk)
         o, d = camera_rays(cam, W, H, 0.69, dev)
         o, d = o.cpu().numpy(), d.cpu().numpy()
-        ref = _oracle(sc, 'baseline', layers, o, d)
+        ref = _oracle(sc, 'baseline', layers, o, d, walk=walk)
         rgb_ref = (ref['rgb'] + ref['T'][:, None]).reshape(H, W, 3)
         p = psnr(rgb.cpu().numpy(), rgb_ref)
         err = np.abs(rgb.cpu().numpy() - rgb_ref).max()
@@ -72,24 +75,41 @@ def test_render_rays_reference_decoder_fp32(scene64, cameras, dev, hidden):
     grid = SceneGrid.from_synthetic(sc, dev)
     o, d = S.camera_rays(cameras[3], 64, 64)
     rgb, T, cnt = render_rays(grid, dec, torch.from_numpy(o).to(dev), torch.from_numpy(d).to(dev),
-                              'fp32', t_stop=1e-4)
-    ref = _oracle(sc, 'reference', sd, o, d)
+                              'fp32', t_stop=1e-4, traversal='offline')
+    ref = _oracle(sc, 'reference', sd, o, d, walk='offline')
     rgb_ref = ref['rgb'] + ref['T'][:, None]
     assert np.abs(rgb.cpu().numpy() - rgb_ref).max() < 1e-4
     assert _count_ok(cnt.cpu().numpy(), ref['used']) >= 0.999
 
 
-def test_render_without_termination_uses_every_interval(scene64, cameras, dev):
+@pytest.mark.parametrize('walk', ['realtime', 'offline'])
+def test_render_without_termination_uses_every_interval(scene64, cameras, dev, walk):
     sc = scene64
     layers = S.baseline_decoder_layers(32, 32)
     dec = Decoder(from_baseline(layers, 32))
     grid = SceneGrid.from_synthetic(sc, dev)
     o, d = S.camera_rays(cameras[4], 64, 64)
     rgb, T, cnt = render_rays(grid, dec, torch.from_numpy(o).to(dev), torch.from_numpy(d).to(dev),
-                              'fp32', t_stop=0.0)
-    ref = _oracle(sc, 'baseline', layers, o, d, t_stop=0.0)
+                              'fp32', t_stop=0.0, traversal=walk)
+    ref = _oracle(sc, 'baseline', layers, o, d, t_stop=0.0, walk=walk)
     assert np.array_equal(cnt.cpu().numpy(), ref['num'])
     assert np.abs(T.cpu().numpy() - ref['T']).max() < 1e-5
+
+
+def test_render_alpha_skip_reference_mode(scene64, cameras, dev):
+    """mlp_eval's alpha skip (mlp_eval.cu:153-155) reproduced by the fused kernel."""
+    sc = scene64
+    sd = S.reference_decoder_state(32, 64)
+    dec = Decoder(from_state_dict(sd, 32))
+    grid = SceneGrid.from_synthetic(sc, dev)
+    o, d = S.camera_rays(cameras[9], 64, 64)
+    rgb, T, cnt = render_rays(grid, dec, torch.from_numpy(o).to(dev), torch.from_numpy(d).to(dev),
+                              'fp32', t_stop=1e-2, alpha_skip=1e-2)
+    ref = oracle_render(sc.dense_voxels(), sc.mask, sc.n, sc.xyzmin, sc.voxel_size,
+                        lambda F, dd: fp64.mlp_chain(sd, F, dd), 0, o, d, 1e-2, alpha_skip=1e-2,
+                        walk='realtime')
+    err = np.abs(rgb.cpu().numpy() - (ref['rgb'] + ref['T'][:, None])).max(1)
+    assert np.mean(err < 1e-4) >= 0.99
 
 
 def test_render_256_subsample(cameras, dev):
@@ -101,7 +121,7 @@ def test_render_256_subsample(cameras, dev):
     o, d = S.camera_rays(cameras[8], 800, 800)
     sel = np.arange(0, o.shape[0], 37)
     o, d = o[sel], d[sel]
-    ref = _oracle(sc, 'baseline', layers, o, d)
+    ref = _oracle(sc, 'baseline', layers, o, d, walk='realtime')
     rgb_ref = ref['rgb'] + ref['T'][:, None]
     for precision in ('fp32', 'bf16'):
         rgb, T, cnt = render_rays(grid, dec, torch.from_numpy(o).to(dev),

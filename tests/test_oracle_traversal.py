@@ -188,6 +188,43 @@ def test_world_units_and_mask_scale():
     assert np.mean(na == nb) > 0.97
 
 
+def test_rt_walk_equals_aabb_plus_march_rounds(scene64, cameras):
+    """oracle_rt_walk (one pass per ray) == aabb_intersect followed by 8-interval ray_march
+    rounds, bit for bit: the rounds do not change the recurrence."""
+    from diver_amd import synthetic as S
+    sc = scene64
+    n = sc.n
+    o, d = S.camera_rays(cameras[7], 32, 32)
+    B = o.shape[0]
+    octrees = oracle.octree_levels(sc.mask)
+    mask = sc.mask.astype(np.uint8)
+    walk, num = oracle.rt_walk(o, d, sc.xyzmin, sc.voxel_size, n, octrees, mask)
+    centre = ((o[0] - np.float32(sc.xyzmin)) / np.float32(sc.voxel_size)).astype(np.float32)
+    coords = np.zeros((8, B, 6), np.float32)
+    finish = np.zeros(B, np.uint8)
+    rgba = np.zeros((B, 4), np.float32)
+    oracle.aabb_intersect(coords, d, centre, finish, rgba, octrees, n)
+    rows = [[] for _ in range(B)]
+    fin = finish.copy()
+    for _ in range(200):
+        if fin.all():
+            break
+        oracle.ray_march(coords, d, mask, fin, n)
+        for b in range(B):
+            if fin[b]:
+                continue
+            for h in range(8):
+                if coords[h, b, 0] < 0:
+                    fin[b] = 1
+                    break
+                rows[b].append(coords[h, b].copy())
+    assert fin.all() and num.sum() > 0
+    for b in range(B):
+        got = np.array(rows[b], np.float32).reshape(-1, 6)
+        assert len(got) == num[b]
+        assert np.array_equal(got, walk[b, :num[b]])
+
+
 def test_real_time_march_covers_masked_traversal(scene64, cameras):
     """aabb_intersect + ray_march rounds emit the same occupied intervals as masked_intersect
     from the first occupied N/4 block on (ray_march.cu / ray_voxel_intersect.cu share the DDA)."""
