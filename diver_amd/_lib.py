@@ -18,7 +18,7 @@ EXPORTS = [
     'diver_aabb_intersect', 'diver_ray_march', 'diver_integrate', 'diver_integrate_backward',
     'diver_composite', 'diver_composite_backward', 'diver_decoder_create', 'diver_decoder_destroy',
     'diver_decoder_param_count', 'diver_mlp_eval', 'diver_render_rays', 'diver_render_camera',
-    'diver_camera_rays',
+    'diver_camera_rays', 'diver_block_bits',
 ]
 
 
@@ -40,7 +40,7 @@ class DecoderDesc(ctypes.Structure):
 
 class Grid(ctypes.Structure):
     _fields_ = [('voxels', ctypes.c_void_p), ('mask', ctypes.c_void_p),
-                ('octrees', ctypes.c_void_p), ('n', ctypes.c_int),
+                ('octrees', ctypes.c_void_p), ('blocks', ctypes.c_void_p), ('n', ctypes.c_int),
                 ('channels', ctypes.c_int), ('xyzmin', ctypes.c_float),
                 ('voxel_size', ctypes.c_float)]
 
@@ -74,6 +74,7 @@ def _declare(L):
         'diver_render_camera': [ctypes.POINTER(Grid), vp, ctypes.POINTER(RenderOpts),
                                 ctypes.POINTER(ctypes.c_float), i32, i32, fl, vp, vp, vp],
         'diver_camera_rays': [ctypes.POINTER(ctypes.c_float), i32, i32, fl, vp, vp, vp],
+        'diver_block_bits': [vp, i32, vp, vp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

@@ -175,11 +175,29 @@ PRECISIONS = {'fp32': 0, 'bf16': 1, 'fp16': 2}
 TRAVERSALS = {'offline': 0, 'realtime': 1}
 
 
-def grid_struct(voxels, mask, octrees, xyzmin, voxel_size):
+def grid_struct(voxels, mask, octrees, xyzmin, voxel_size, blocks=None):
     n = mask.shape[0]
     return _lib.Grid(voxels.data_ptr(), mask.data_ptr(),
-                     octrees.data_ptr() if octrees is not None else None, int(n),
+                     octrees.data_ptr() if octrees is not None else None,
+                     blocks.data_ptr() if blocks is not None else None, int(n),
                      int(voxels.shape[-1]), float(xyzmin), float(voxel_size))
+
+
+def block_bits(mask):
+    """8^3-block occupancy bits of an [n^3] mask (int32 words, bit b of word b // 32)."""
+    _dev(mask)
+    mask = _u8(mask, 'mask')
+    n = mask.shape[0]
+    nb = n // 8
+    words = (nb ** 3 + 31) // 32
+    bits = torch.empty(words, dtype=torch.int32, device=mask.device)
+    _lib.call('diver_block_bits', mask.data_ptr(), int(n), bits.data_ptr(), _s(mask))
+    return bits
+
+
+def _grid_of(grid):
+    return grid_struct(grid.voxels, grid.mask, grid.octrees, grid.xyzmin, grid.voxel_size,
+                       getattr(grid, 'blocks', None))
 
 
 def render_opts(precision='fp32', traversal='realtime', t_stop=1e-4, alpha_skip=0.0):
@@ -194,7 +212,7 @@ def render_rays(grid, decoder, o, d, opts, with_count=True):
     R = o.shape[0]
     rgba = torch.empty((R, 4), dtype=torch.float32, device=dev)
     count = torch.empty(R, dtype=torch.int32, device=dev) if with_count else None
-    g = grid_struct(grid.voxels, grid.mask, grid.octrees, grid.xyzmin, grid.voxel_size)
+    g = _grid_of(grid)
     _lib.call('diver_render_rays', ctypes.byref(g), decoder.handle, ctypes.byref(opts), R,
               o.data_ptr(), d.data_ptr(), rgba.data_ptr(), _lib.ptr(count), _s(o))
     return rgba, count
@@ -205,7 +223,7 @@ def render_camera(grid, decoder, c2w, width, height, focal, opts, with_count=Tru
     rgba = out if out is not None else torch.empty((height, width, 4), dtype=torch.float32,
                                                    device=dev)
     count = torch.empty((height, width), dtype=torch.int32, device=dev) if with_count else None
-    g = grid_struct(grid.voxels, grid.mask, grid.octrees, grid.xyzmin, grid.voxel_size)
+    g = _grid_of(grid)
     m = (ctypes.c_float * 12)(*[float(x) for x in list(c2w.reshape(-1))])
     _lib.call('diver_render_camera', ctypes.byref(g), decoder.handle, ctypes.byref(opts), m,
               int(width), int(height), float(focal), rgba.data_ptr(), _lib.ptr(count),

@@ -73,6 +73,7 @@ struct diver_decoder {
     int variant;
     int device;
     int64_t n_params;
+    bool density_ch0 = false;      // sigma = act(F[0]): w_s = e_0, b_s = 0, no pre-layers
     float* W = nullptr;            // canonical fp32
     uint32_t* pack_bf16 = nullptr; // matrix-core A operands
     uint32_t* pack_f16 = nullptr;
@@ -249,6 +250,9 @@ int diver_decoder_create(const diver_decoder_desc* desc, const float* params_hos
 
     if (desc->n_pre == 0) {
         const int C = desc->channels, H = desc->hidden;
+        bool ch0 = params_host[0] == 1.0f && params_host[C] == 0.0f;
+        for (int i = 1; i < C; ++i) ch0 = ch0 && params_host[i] == 0.0f;
+        d->density_ch0 = ch0;
         const int64_t wc = (int64_t)C + 1;  // w_s[Z], b_s then W_c
         const int64_t post0 = wc + (int64_t)H * C + (int64_t)H * 27 + H;
         const int64_t outo = post0 + (int64_t)desc->n_post * (H * H + H);
@@ -295,6 +299,21 @@ int diver_mlp_eval(const diver_decoder* dec, int64_t batch, const float* coord,
                       "mlp_eval");
 }
 
+// Decoders without pre-layers use the queue kernel (per-lane walk + density, batched colour)
+// when the block bitmap is available; everything else uses the round kernel.
+static hipError_t dispatch_render(const diver_decoder* dec, const diver::RenderArgs& a,
+                                  int precision, hipStream_t s) {
+    if (dec->desc.n_pre == 0 && a.blocks)
+        return diver::launch_render_queue(dec->variant, a, precision, dec->density_ch0, s);
+    return diver::launch_render(dec->variant, a, precision, s);
+}
+
+int diver_block_bits(const uint8_t* mask, int n, uint32_t* bits, diver_stream_t stream) {
+    REQUIRE(mask && bits, "null pointer");
+    REQUIRE(n >= 8 && n % 8 == 0, "n must be a positive multiple of 8");
+    return hip_status(diver::launch_block_bits(mask, n, bits, S(stream)), "block_bits");
+}
+
 static int render_common(const diver_grid* g, const diver_decoder* dec,
                          const diver_render_opts* opts, diver::RenderArgs& a) {
     REQUIRE(g && dec && opts, "null grid, decoder or options");
@@ -319,6 +338,8 @@ static int render_common(const diver_grid* g, const diver_decoder* dec,
     a.voxels = g->voxels;
     a.mask = g->mask;
     a.octrees = g->octrees;
+    a.blocks = g->blocks;
+    if (g->blocks) REQUIRE(g->n % 8 == 0, "block bits need n to be a multiple of 8");
     a.traversal = opts->traversal;
     a.n = g->n;
     a.channels = g->channels;
@@ -345,7 +366,7 @@ int diver_render_rays(const diver_grid* grid, const diver_decoder* dec,
     a.cam.width = 0;
     a.rgba = rgba;
     a.count = count;
-    return hip_status(diver::launch_render(dec->variant, a, opts->precision, S(stream)),
+    return hip_status(dispatch_render(dec, a, opts->precision, S(stream)),
                       "render_rays");
 }
 
@@ -366,7 +387,7 @@ int diver_render_camera(const diver_grid* grid, const diver_decoder* dec,
     a.n_rays = tiles * 64;
     a.rgba = rgba;
     a.count = count;
-    return hip_status(diver::launch_render(dec->variant, a, opts->precision, S(stream)),
+    return hip_status(dispatch_render(dec, a, opts->precision, S(stream)),
                       "render_camera");
 }
 

@@ -16,6 +16,7 @@ struct RenderArgs {
     const float* voxels;   // [(N+1)^3][C] fp32
     const uint8_t* mask;   // [N^3]
     const uint8_t* octrees;  // occupancy pyramid (block sizes 64, 16, 4), real-time traversal
+    const uint32_t* blocks;  // 8^3-block occupancy bits (optional)
     int traversal;         // 0: masked_intersect walk (exit > N); 1: real-time walk
     int n;
     int channels;
@@ -61,6 +62,9 @@ hipError_t launch_mlp_eval(int variant, int64_t batch, const float* coord, const
                            uint8_t* finish, const float* W, float alpha_skip, float t_stop,
                            hipStream_t s);
 hipError_t launch_render(int variant, const RenderArgs& a, int prec, hipStream_t s);
+hipError_t launch_render_queue(int variant, const RenderArgs& a, int prec, bool dch0,
+                               hipStream_t s);
+hipError_t launch_block_bits(const uint8_t* mask, int n, uint32_t* bits, hipStream_t s);
 hipError_t launch_camera_rays(const CameraDesc& cam, float* o, float* d, hipStream_t s);
 
 }  // namespace diver

@@ -30,6 +30,7 @@ class SceneGrid:
         self.voxel_size = float(voxel_size)
         self.octrees = (build_octrees(self.mask.bool()).to(torch.uint8).contiguous()
                         if n % 64 == 0 else None)
+        self.blocks = calls.block_bits(self.mask) if n % 8 == 0 else None
 
     @classmethod
     def from_synthetic(cls, scene, device) -> 'SceneGrid':

@@ -150,6 +150,8 @@ typedef struct diver_grid {
     const uint8_t* mask;    /* [n^3] */
     const uint8_t* octrees; /* occupancy pyramid for the real-time walk (block sizes 64, 16, 4,
                                coarse first, eval_rt.py:236-239); NULL for the offline walk */
+    const uint32_t* blocks; /* 8^3-block occupancy bits from diver_block_bits, or NULL; lets
+                               empty blocks skip the fine mask (no effect on results) */
     int n;
     int channels;
     float xyzmin;
@@ -185,6 +187,12 @@ int diver_render_camera(const diver_grid* grid, const diver_decoder* dec,
                         const diver_render_opts* opts, const float* c2w_host, int width,
                         int height, float focal, float* rgba, int32_t* count,
                         diver_stream_t stream);
+
+/* 8^3-block occupancy bits of an [n^3] mask (n a multiple of 8): bit b of word b/32 is set
+ * when block b = (bz*(n/8) + by)*(n/8) + bx holds an occupied voxel; bits has
+ * ceil((n/8)^3 / 32) words.  No counterpart in the reference (an acceleration structure for
+ * diver_render_*). */
+int diver_block_bits(const uint8_t* mask, int n, uint32_t* bits, diver_stream_t stream);
 
 /* The FP32 rays diver_render_camera traces: o, d [height*width][3]. */
 int diver_camera_rays(const float* c2w_host, int width, int height, float focal, float* o,
