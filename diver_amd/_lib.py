@@ -18,7 +18,7 @@ EXPORTS = [
     'diver_aabb_intersect', 'diver_ray_march', 'diver_integrate', 'diver_integrate_backward',
     'diver_composite', 'diver_composite_backward', 'diver_decoder_create', 'diver_decoder_destroy',
     'diver_decoder_param_count', 'diver_mlp_eval', 'diver_render_rays', 'diver_render_camera',
-    'diver_camera_rays', 'diver_block_bits',
+    'diver_camera_rays', 'diver_block_bits', 'diver_walk_rays',
 ]
 
 
@@ -75,6 +75,7 @@ def _declare(L):
                                 ctypes.POINTER(ctypes.c_float), i32, i32, fl, vp, vp, vp],
         'diver_camera_rays': [ctypes.POINTER(ctypes.c_float), i32, i32, fl, vp, vp, vp],
         'diver_block_bits': [vp, i32, vp, vp],
+        'diver_walk_rays': [ctypes.POINTER(Grid), i32, i64, vp, vp, i32, vp, vp, vp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

@@ -205,6 +205,20 @@ def render_opts(precision='fp32', traversal='realtime', t_stop=1e-4, alpha_skip=
                            float(alpha_skip))
 
 
+def walk_rays(grid, o, d, traversal='realtime', max_n=None):
+    """The occupied intervals the fused renderer composites: (isect [R,max_n,6], num [R])."""
+    dev = _dev(grid.voxels, grid.mask, o, d)
+    o, d = _f32(o, 'o'), _f32(d, 'd')
+    R = o.shape[0]
+    max_n = 3 * grid.n if max_n is None else max_n
+    isect = torch.empty((R, max_n, 6), dtype=torch.float32, device=dev)
+    num = torch.empty(R, dtype=torch.int32, device=dev)
+    g = _grid_of(grid)
+    _lib.call('diver_walk_rays', ctypes.byref(g), TRAVERSALS[traversal], R, o.data_ptr(),
+              d.data_ptr(), int(max_n), isect.data_ptr(), num.data_ptr(), _s(o))
+    return isect, num
+
+
 def render_rays(grid, decoder, o, d, opts, with_count=True):
     """grid: an object with voxels, mask, octrees, xyzmin, voxel_size (renderer.SceneGrid)."""
     dev = _dev(grid.voxels, grid.mask, o, d)

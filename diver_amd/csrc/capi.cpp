@@ -308,6 +308,30 @@ static hipError_t dispatch_render(const diver_decoder* dec, const diver::RenderA
     return diver::launch_render(dec->variant, a, precision, s);
 }
 
+int diver_walk_rays(const diver_grid* g, int traversal, int64_t n_rays, const float* o,
+                    const float* d, int max_n, float* isect, int32_t* num,
+                    diver_stream_t stream) {
+    REQUIRE(g && g->mask && n_rays >= 0 && max_n >= 0, "bad arguments");
+    REQUIRE(n_rays == 0 || (o && d && num && (max_n == 0 || isect)), "null pointer");
+    REQUIRE(traversal == DIVER_WALK_OFFLINE || traversal == DIVER_WALK_REALTIME,
+            "bad traversal %d", traversal);
+    if (traversal == DIVER_WALK_REALTIME)
+        REQUIRE(g->octrees && g->n % 64 == 0, "the real-time walk needs the pyramid, n % 64 == 0");
+    REQUIRE(g->blocks && g->n % 8 == 0, "the walk needs the block bits (n % 8 == 0)");
+    diver::RenderArgs a{};
+    a.mask = g->mask;
+    a.octrees = g->octrees;
+    a.blocks = g->blocks;
+    a.traversal = traversal;
+    a.n = g->n;
+    a.xyzmin = g->xyzmin;
+    a.voxel_size = g->voxel_size;
+    a.n_rays = n_rays;
+    a.o = o;
+    a.d = d;
+    return hip_status(diver::launch_walk(a, max_n, isect, num, S(stream)), "walk_rays");
+}
+
 int diver_block_bits(const uint8_t* mask, int n, uint32_t* bits, diver_stream_t stream) {
     REQUIRE(mask && bits, "null pointer");
     REQUIRE(n >= 8 && n % 8 == 0, "n must be a positive multiple of 8");

@@ -81,6 +81,45 @@ __device__ __forceinline__ float dda_step(float px, float py, float pz, float dx
     return tn;
 }
 
+// x / d, correctly rounded, from y = RN(1/d): q = RN(x*y), r = x - q*d (exact by FMA),
+// q' = RN(q + r*y) (Markstein's correction).  Valid for normal d with |1/d| finite and moderate
+// |x|; callers use it only for direction components with |d| >= kMinFastDir.
+constexpr float kMinFastDir = 1e-30f;
+__device__ __forceinline__ float div_recip(float x, float d, float y) {
+#pragma clang fp contract(off)
+    float q = x * y;
+    float r = __builtin_fmaf(-q, d, x);
+    return __builtin_fmaf(r, y, q);
+}
+
+// dda_step with the three divisions done by div_recip (bit-identical to dda_step).
+__device__ __forceinline__ float dda_step_r(float px, float py, float pz, float dx, float dy,
+                                            float dz, float rx, float ry, float rz, float sx,
+                                            float sy, float sz, float& qx, float& qy, float& qz) {
+#pragma clang fp contract(off)
+    float bx = floorf(px * sx + 1.0f) * sx;
+    float by = floorf(py * sy + 1.0f) * sy;
+    float bz = floorf(pz * sz + 1.0f) * sz;
+    float tx = div_recip(bx - px, dx, rx);
+    float ty = div_recip(by - py, dy, ry);
+    float tz = div_recip(bz - pz, dz, rz);
+    float tn = fminf(tx, fminf(ty, tz));
+    float nx = __builtin_fmaf(dx, tn, px);
+    float ny = __builtin_fmaf(dy, tn, py);
+    float nz = __builtin_fmaf(dz, tn, pz);
+    if (tn == tx) {
+        nx = bx;
+    } else if (tn == ty) {
+        ny = by;
+    } else {
+        nz = bz;
+    }
+    qx = nx;
+    qy = ny;
+    qz = nz;
+    return tn;
+}
+
 // Voxel of the segment [l, p]: trunc(min(l + 1e-4, p + 1e-4) * scale) per axis.
 __device__ __forceinline__ void segment_voxel(float lx, float ly, float lz, float px, float py,
                                               float pz, float scale, int& vx, int& vy, int& vz) {

@@ -9,19 +9,41 @@ namespace diver {
 struct RayWalk {
     float lx, ly, lz;  // current point (grid units)
     float dx, dy, dz;
+    float rx, ry, rz;  // RN(1/d) per axis, for div_recip
     float sx, sy, sz;
     int steps;
     int max_steps;
     bool alive;
+    bool fast;         // every |d| >= kMinFastDir: divisions by reciprocal + correction
+
+    __device__ __forceinline__ void set_dir(float dx_, float dy_, float dz_) {
+        dx = dx_;
+        dy = dy_;
+        dz = dz_;
+        rx = 1.0f / dx;
+        ry = 1.0f / dy;
+        rz = 1.0f / dz;
+        fast = fabsf(dx) >= kMinFastDir && fabsf(dy) >= kMinFastDir && fabsf(dz) >= kMinFastDir;
+        sx = (float)sgnf(dx);
+        sy = (float)sgnf(dy);
+        sz = (float)sgnf(dz);
+    }
+
+    // one exact DDA step from (px,py,pz)
+    __device__ __forceinline__ void step(float px, float py, float pz, float& qx, float& qy,
+                                         float& qz) const {
+        if (fast)
+            dda_step_r(px, py, pz, dx, dy, dz, rx, ry, rz, sx, sy, sz, qx, qy, qz);
+        else
+            dda_step(px, py, pz, dx, dy, dz, sx, sy, sz, qx, qy, qz);
+    }
 
     // World-space ray -> grid units, box entry as masked_intersect does it.
     __device__ __forceinline__ void init(float ox, float oy, float oz, float dx_, float dy_,
                                          float dz_, float xyzmin, float voxel_size, float n,
                                          bool valid) {
 #pragma clang fp contract(off)
-        dx = dx_;
-        dy = dy_;
-        dz = dz_;
+        set_dir(dx_, dy_, dz_);
         float gx = (ox - xyzmin) / voxel_size;
         float gy = (oy - xyzmin) / voxel_size;
         float gz = (oz - xyzmin) / voxel_size;
@@ -34,9 +56,6 @@ struct RayWalk {
             float tmin;
             alive = box_entry(gx, gy, gz, dx, dy, dz, n, lx, ly, lz, tmin);
         }
-        sx = (float)sgnf(dx);
-        sy = (float)sgnf(dy);
-        sz = (float)sgnf(dz);
         steps = 0;
         max_steps = max_dda_steps(n);
     }
@@ -50,12 +69,7 @@ struct RayWalk {
                                             const uint8_t* __restrict__ octrees, bool valid) {
 #pragma clang fp contract(off)
         const float nf = (float)n;
-        dx = dx_;
-        dy = dy_;
-        dz = dz_;
-        sx = (float)sgnf(dx);
-        sy = (float)sgnf(dy);
-        sz = (float)sgnf(dz);
+        set_dir(dx_, dy_, dz_);
         steps = 0;
         max_steps = max_dda_steps(nf);
         alive = false;
@@ -80,7 +94,7 @@ struct RayWalk {
             qx = px;
             qy = py;
             qz = pz;
-            dda_step(qx, qy, qz, dx, dy, dz, sx, sy, sz, px, py, pz);
+            step(qx, qy, qz, px, py, pz);
             if (outside(px, py, pz, (float)m)) return;
             int vx, vy, vz;
             segment_voxel1(qx, qy, qz, px, py, pz, vx, vy, vz);
@@ -116,7 +130,7 @@ struct RayWalk {
         while (alive && steps < max_steps) {
             ++steps;
             float nx, ny, nz;
-            dda_step(lx, ly, lz, dx, dy, dz, sx, sy, sz, nx, ny, nz);
+            step(lx, ly, lz, nx, ny, nz);
             if (OPEN ? outside_open(nx, ny, nz, nf) : outside(nx, ny, nz, nf)) break;
             int vx, vy, vz;
             segment_voxel1(lx, ly, lz, nx, ny, nz, vx, vy, vz);

@@ -176,8 +176,7 @@ __global__ void __launch_bounds__(256) render_queue_kernel(RenderArgs a) {
                 } else {
                     ++ray.steps;
                     float nx, ny, nz;
-                    dda_step(ray.lx, ray.ly, ray.lz, ray.dx, ray.dy, ray.dz, ray.sx, ray.sy,
-                             ray.sz, nx, ny, nz);
+                    ray.step(ray.lx, ray.ly, ray.lz, nx, ny, nz);
                     bool out = RT ? outside_open(nx, ny, nz, nf) : outside(nx, ny, nz, nf);
                     if (out) {
                         ray.alive = false;
@@ -335,6 +334,62 @@ hipError_t launch_render_queue(int variant, const RenderArgs& a, int prec, bool 
         case 2: return launch_q1<DecB64C64>(a, prec, dch0, s);
         default: return hipErrorInvalidValue;
     }
+}
+
+// The fused kernels' walk on its own: every occupied interval of each ray, in masked_intersect's
+// layout ([rays][max_n][6], count per ray).  Same step, entry and occupancy code as the queue
+// kernel's phase A, so its output is what the fused renderer composites.
+template <bool RT>
+__global__ void __launch_bounds__(256) walk_kernel(RenderArgs a, int max_n, float* __restrict__ isect,
+                                                   int32_t* __restrict__ num) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= a.n_rays) return;
+    const int n = a.n;
+    const int nb = n >> 3;
+    RayWalk ray;
+    if constexpr (RT)
+        ray.init_rt(a.o[3 * r], a.o[3 * r + 1], a.o[3 * r + 2], a.d[3 * r], a.d[3 * r + 1],
+                    a.d[3 * r + 2], a.xyzmin, a.voxel_size, n, a.octrees, true);
+    else
+        ray.init(a.o[3 * r], a.o[3 * r + 1], a.o[3 * r + 2], a.d[3 * r], a.d[3 * r + 1],
+                 a.d[3 * r + 2], a.xyzmin, a.voxel_size, (float)n, true);
+    const float nf = (float)n;
+    int count = 0;
+    while (ray.alive && ray.steps < ray.max_steps) {
+        ++ray.steps;
+        float nx, ny, nz;
+        ray.step(ray.lx, ray.ly, ray.lz, nx, ny, nz);
+        if (RT ? outside_open(nx, ny, nz, nf) : outside(nx, ny, nz, nf)) break;
+        int vx, vy, vz;
+        segment_voxel1(ray.lx, ray.ly, ray.lz, nx, ny, nz, vx, vy, vz);
+        if (voxel_occupied(a.mask, a.blocks, n, nb, vx, vy, vz)) {
+            if (count < max_n) {
+                float* o = isect + (r * max_n + count) * 6;
+                o[0] = ray.lx; o[1] = ray.ly; o[2] = ray.lz;
+                o[3] = nx; o[4] = ny; o[5] = nz;
+            }
+            ++count;
+        }
+        ray.lx = nx;
+        ray.ly = ny;
+        ray.lz = nz;
+    }
+    num[r] = count;
+    for (int k = count; k < max_n; ++k) {
+        float* o = isect + (r * max_n + k) * 6;
+        o[0] = 0.f; o[1] = 0.f; o[2] = 0.f; o[3] = 0.f; o[4] = 0.f; o[5] = 0.f;
+    }
+}
+
+hipError_t launch_walk(const RenderArgs& a, int max_n, float* isect, int32_t* num,
+                       hipStream_t s) {
+    if (a.n_rays == 0) return hipSuccess;
+    dim3 grid((unsigned)((a.n_rays + 255) / 256)), block(256);
+    if (a.traversal == 1)
+        hipLaunchKernelGGL(walk_kernel<true>, grid, block, 0, s, a, max_n, isect, num);
+    else
+        hipLaunchKernelGGL(walk_kernel<false>, grid, block, 0, s, a, max_n, isect, num);
+    return hipGetLastError();
 }
 
 __global__ void block_bits_kernel(const uint8_t* __restrict__ mask, int n,

@@ -194,6 +194,16 @@ int diver_render_camera(const diver_grid* grid, const diver_decoder* dec,
  * diver_render_*). */
 int diver_block_bits(const uint8_t* mask, int n, uint32_t* bits, diver_stream_t stream);
 
+/* The occupied intervals the fused renderer composites, one walk per ray (diver_traversal):
+ * the offline walk equals masked_intersect (utils/ray_voxel_intersection/src/
+ * ray_voxel_intersect.cu:192-319); the real-time walk equals aabb_intersect followed by ray_march
+ * rounds until the ray leaves (utils/ray_march/src/ray_march.cu:24-230), each ray from its own
+ * origin.  Same step and occupancy code as diver_render_*; output as diver_masked_intersect
+ * (isect [n_rays][max_n][6], num [n_rays]).  grid->blocks is required. */
+int diver_walk_rays(const diver_grid* grid, int traversal, int64_t n_rays, const float* o,
+                    const float* d, int max_n, float* isect, int32_t* num,
+                    diver_stream_t stream);
+
 /* The FP32 rays diver_render_camera traces: o, d [height*width][3]. */
 int diver_camera_rays(const float* c2w_host, int width, int height, float focal, float* o,
                       float* d, diver_stream_t stream);

@@ -159,6 +159,41 @@ def test_aabb_and_ray_march_bit_exact(scene256, cameras, dev):
     assert _rt_rounds(scene256, o, d, dev) >= 2
 
 
+@pytest.mark.parametrize('walk', ['realtime', 'offline'])
+def test_fused_walk_bit_exact(scene256, cameras, dev, walk):
+    """The fused renderer's walk (reciprocal-corrected divisions, block-bitmap skipping) emits
+    exactly the reference's intervals: masked_intersect for the offline walk, aabb_intersect +
+    ray_march rounds for the real-time walk (each ray from its own origin)."""
+    from diver_amd.renderer import SceneGrid
+    sc = scene256
+    grid = SceneGrid.from_synthetic(sc, dev)
+    rng = np.random.default_rng(17)
+    o, d = _sub_rays(cameras[5], 800, 800, 6)
+    # plus rays from random origins, inside and outside the grid, and axis-aligned components
+    o2 = rng.uniform(-2.5, 2.5, size=(4000, 3)).astype(np.float32)
+    o2[:800] = rng.uniform(-0.9, 0.9, size=(800, 3))
+    d2 = rng.uniform(-1, 1, size=(4000, 3)) - 0.3 * o2
+    d2 /= np.linalg.norm(d2, axis=1, keepdims=True)
+    d2 = d2.astype(np.float32)
+    d2[:40, 0] = 0.0
+    d2[40:80, 2] = -0.0
+    o = np.concatenate([o, o2]).astype(np.float32)
+    d = np.concatenate([d, d2]).astype(np.float32)
+    max_n = 512
+    if walk == 'realtime':
+        ri, rn = oracle.rt_walk(o, d, sc.xyzmin, sc.voxel_size, sc.n,
+                                oracle.octree_levels(sc.mask), sc.mask, max_n)
+    else:
+        vs = np.float32(sc.voxel_size)
+        ri, rn, _ = oracle.masked_intersect(o, d, np.float32(sc.xyzmin), np.float32(1), sc.n, vs,
+                                            sc.mask, 1.0, max_n=max_n)
+    gi, gn = calls.walk_rays(grid, torch.from_numpy(o).to(dev), torch.from_numpy(d).to(dev),
+                             walk, max_n)
+    assert rn.max() <= max_n and rn.sum() > 10000
+    assert np.array_equal(gn.cpu().numpy(), rn)
+    assert np.array_equal(gi.cpu().numpy(), ri)
+
+
 def test_ray_march_grid_stride_quirk(scene64, cameras, dev):
     """A resumable ray whose resume point is negative ends its thread's sweep
     (ray_march.cu:171-173): with a small stride the later rays of that thread are skipped."""
