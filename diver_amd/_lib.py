@@ -18,7 +18,8 @@ EXPORTS = [
     'diver_aabb_intersect', 'diver_ray_march', 'diver_integrate', 'diver_integrate_backward',
     'diver_composite', 'diver_composite_backward', 'diver_decoder_create', 'diver_decoder_destroy',
     'diver_decoder_param_count', 'diver_mlp_eval', 'diver_render_rays', 'diver_render_camera',
-    'diver_camera_rays', 'diver_block_bits', 'diver_walk_rays',
+    'diver_camera_rays', 'diver_block_bits', 'diver_walk_rays', 'diver_render_rows',
+    'diver_decoder_update',
 ]
 
 
@@ -68,6 +69,7 @@ def _declare(L):
         'diver_decoder_create': [ctypes.POINTER(DecoderDesc), ctypes.POINTER(ctypes.c_float), i64,
                                  ctypes.POINTER(vp)],
         'diver_decoder_destroy': [vp],
+        'diver_decoder_update': [vp, vp, i64, vp],
         'diver_mlp_eval': [vp, i64, vp, vp, vp, i32, vp, vp, vp, fl, fl, vp],
         'diver_render_rays': [ctypes.POINTER(Grid), vp, ctypes.POINTER(RenderOpts), i64, vp, vp,
                               vp, vp, vp],
@@ -76,6 +78,8 @@ def _declare(L):
         'diver_camera_rays': [ctypes.POINTER(ctypes.c_float), i32, i32, fl, vp, vp, vp],
         'diver_block_bits': [vp, i32, vp, vp],
         'diver_walk_rays': [ctypes.POINTER(Grid), i32, i64, vp, vp, i32, vp, vp, vp],
+        'diver_render_rows': [ctypes.POINTER(Grid), vp, ctypes.POINTER(RenderOpts), i64, vp, vp,
+                              vp, vp, vp, vp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

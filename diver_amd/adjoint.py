@@ -1,0 +1,168 @@
+"""Differentiable fused render: a hand-derived adjoint behind ``torch.autograd.Function``.
+
+Forward: one fused kernel per batch of rays (FP32), keeping only (rgb, T) and the interval
+count per ray -- no samples are stored for the backward.
+
+Backward (for decoders without pre-layers, e.g. the benchmark decoder):
+  1. re-walk every ray and recompute each composited interval's integral, density and
+     transmittance (``diver_render_rows``);
+  2. compositing adjoint, per interval k of a ray with C = sum_i T_i a_i c_i + bg T_n:
+        dC/dc_k     = T_k a_k
+        dC/dsigma_k = T_{k+1} c_k - (sum_{i>k} T_i a_i c_i + bg T_n)
+     (the suffix sums come from a float64 segmented cumsum over the rows);
+  3. decoder adjoint as batched FP32 GEMMs over the rows (canonical form, include/diver.h):
+        do = dc * c (1 - c);  dW_o = do^T r2;  dr2 = (do W_o) * [y2 > 0];  dW_2 = dr2^T r1
+        dr1 = (dr2 W_2) * [y1 > 0];  dW_c = dr1^T F;  dW_v = dr1^T PE(d);  dF = dr1 W_c + ds w_s
+  4. grid adjoint: ``diver_integrate_backward`` scatters dF through the 8 corner weights.
+The reference obtains the same gradient from torch autograd over stored K x C / B x M
+activations (model/diver.py:141-207, utils/integrator.py:62-113, train.py:102-113).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+import torch.nn as nn
+from torch.autograd import Function
+
+from . import calls
+from .decoder import CanonicalDecoder, Decoder, PE_DIM
+
+
+def positional_encoding(d: torch.Tensor, bands: int = 4) -> torch.Tensor:
+    parts = [d]
+    for l in range(bands):
+        parts.append(torch.sin((2 ** l * math.pi) * d))
+        parts.append(torch.cos((2 ** l * math.pi) * d))
+    return torch.cat(parts, -1)
+
+
+def split_canonical(p: torch.Tensor, C: int, H: int):
+    """Views of the canonical parameter vector of a decoder without pre-layers, one post layer."""
+    sizes = [C, 1, H * C, H * PE_DIM, H, H * H, H, 3 * H, 3]
+    parts = torch.split(p, sizes)
+    ws, bs, wc, wv, bc, w2, b2, wo, bo = parts
+    return (ws, bs, wc.view(H, C), wv.view(H, PE_DIM), bc, w2.view(H, H), b2, wo.view(3, H), bo)
+
+
+class _FusedRender(Function):
+    @staticmethod
+    def forward(ctx, voxels, params, grid, decoder, o, d, t_stop, white_back, traversal):
+        calls.decoder_update(decoder, params)
+        opts = calls.render_opts('fp32', traversal, t_stop, 0.0)
+        rgba, count = calls.render_rays(grid, decoder, o, d, opts)
+        ctx.save_for_backward(params, o, d, rgba, count)
+        ctx.grid, ctx.decoder, ctx.opts = grid, decoder, opts
+        ctx.white_back = white_back
+        ctx.vshape = tuple(voxels.shape)
+        bg = rgba[:, 3:] if white_back else torch.zeros_like(rgba[:, 3:])
+        return rgba[:, :3] + bg
+
+    @staticmethod
+    def backward(ctx, g):
+        params, o, d, rgba, count = ctx.saved_tensors
+        grid, decoder = ctx.grid, ctx.decoder
+        C, H = decoder.channels, decoder.hidden
+        R = o.shape[0]
+        cnt64 = count.to(torch.int64)
+        row_off = torch.cumsum(cnt64, 0) - cnt64
+        K = int(cnt64.sum().item())
+        dev = o.device
+        if K == 0:
+            return torch.zeros(ctx.vshape, device=dev), torch.zeros_like(params), *([None] * 7)
+        calls.decoder_update(decoder, params)
+        rows = calls.render_rows(grid, decoder, o, d, ctx.opts, count, row_off, K)
+        ray = torch.repeat_interleave(torch.arange(R, device=dev), cnt64)
+        pin, pout = rows[:, 0:3], rows[:, 3:6]
+        s, Tb, a, F = rows[:, 6], rows[:, 7], rows[:, 8], rows[:, 9:]
+        ws, bs, Wc, Wv, bc, W2, b2, Wo, bo = split_canonical(params, C, H)
+        pe = positional_encoding(d)[ray]
+        # decoder forward on the rows
+        y1 = F @ Wc.T + pe @ Wv.T + bc
+        r1 = torch.relu(y1)
+        y2 = r1 @ W2.T + b2
+        r2 = torch.relu(y2)
+        c = torch.sigmoid(r2 @ Wo.T + bo)
+        w = Tb * a
+        wc = (w[:, None] * c).double()
+        # suffix sums within each ray (float64 segmented cumsum)
+        cs = torch.cat([torch.zeros(1, 3, dtype=torch.float64, device=dev), torch.cumsum(wc, 0)])
+        incl = cs[1:] - cs[row_off][ray]
+        tot = (cs[row_off + cnt64] - cs[row_off])[ray]
+        bg = rgba[:, 3].double()[ray] if ctx.white_back else torch.zeros_like(Tb, dtype=torch.float64)
+        suffix = (tot - incl + bg[:, None]).float()
+        gk = g.contiguous()[ray]
+        t_after = Tb * (1.0 - a)
+        dsigma = (gk * (t_after[:, None] * c - suffix)).sum(1)
+        act = decoder.canon.density_act
+        ds = dsigma * (torch.sigmoid(s) if act == 1 else (s > 0).to(s.dtype))
+        dc = gk * w[:, None]
+        do = dc * c * (1.0 - c)
+        dWo = do.T @ r2
+        dbo = do.sum(0)
+        dr2 = (do @ Wo) * (y2 > 0)
+        dW2 = dr2.T @ r1
+        db2 = dr2.sum(0)
+        dr1 = (dr2 @ W2) * (y1 > 0)
+        dWc = dr1.T @ F
+        dWv = dr1.T @ pe
+        dbc = dr1.sum(0)
+        dF = dr1 @ Wc + ds[:, None] * ws[None, :]
+        dws = ds @ F
+        dbs = ds.sum().reshape(1)
+        gp = torch.cat([dws, dbs, dWc.reshape(-1), dWv.reshape(-1), dbc, dW2.reshape(-1), db2,
+                        dWo.reshape(-1), dbo])
+        gv = calls.integrate_backward(ctx.vshape, pin, pout, dF.contiguous())
+        return gv, gp, None, None, None, None, None, None, None
+
+
+class BenchmarkDecoder(nn.Module):
+    """Trainable benchmark decoder: density softplus(f[0]); colour MLP on [f[1:], PE(d)]
+    (Linear (C-1+27)->H, ReLU, Linear H->H, ReLU, Linear H->3, sigmoid)."""
+
+    def __init__(self, layers, channels: int):
+        super().__init__()
+        (w1, b1), (w2, b2), (w3, b3) = layers
+        t = lambda x: nn.Parameter(torch.as_tensor(np.asarray(x, np.float32)).clone())
+        self.w1, self.b1, self.w2, self.b2, self.w3, self.b3 = (t(w1), t(b1), t(w2), t(b2),
+                                                                 t(w3), t(b3))
+        self.channels = channels
+        self.hidden = w1.shape[0]
+
+    def canonical(self) -> torch.Tensor:
+        C, H = self.channels, self.hidden
+        dev = self.w1.device
+        ws = torch.zeros(C, device=dev)
+        ws[0] = 1.0
+        wc = torch.cat([torch.zeros(H, 1, device=dev), self.w1[:, :C - 1]], 1)
+        wv = self.w1[:, C - 1:]
+        return torch.cat([ws, torch.zeros(1, device=dev), wc.reshape(-1), wv.reshape(-1), self.b1,
+                          self.w2.reshape(-1), self.b2, self.w3.reshape(-1), self.b3])
+
+    def layers(self):
+        f = lambda x: x.detach().cpu().numpy()
+        return [(f(self.w1), f(self.b1)), (f(self.w2), f(self.b2)), (f(self.w3), f(self.b3))]
+
+
+class FusedRenderer(nn.Module):
+    """Trainable grid + benchmark decoder rendered by the fused kernel with the hand-written
+    adjoint.  ``voxels`` [(N+1)^3, C] and the decoder are parameters; the occupancy mask (and its
+    acceleration structures) is fixed."""
+
+    def __init__(self, grid, decoder_module: BenchmarkDecoder, traversal: str = 'offline'):
+        super().__init__()
+        self.voxels = nn.Parameter(grid.voxels)
+        grid.voxels = self.voxels.data  # the kernels read the parameter's storage
+        self.grid = grid
+        self.decoder = decoder_module
+        canon = CanonicalDecoder(decoder_module.channels, decoder_module.hidden, 0, False, 1, 1,
+                                 decoder_module.canonical().detach().cpu().numpy())
+        self.handle = Decoder(canon)
+        self.traversal = traversal
+
+    def forward(self, o, d, t_stop: float = 1e-4, white_back: bool = True):
+        self.grid.voxels = self.voxels.data
+        return _FusedRender.apply(self.voxels, self.decoder.canonical(), self.grid, self.handle,
+                                  o.contiguous(), d.contiguous(), t_stop, white_back,
+                                  self.traversal)

@@ -219,6 +219,29 @@ def walk_rays(grid, o, d, traversal='realtime', max_n=None):
     return isect, num
 
 
+def render_rows(grid, decoder, o, d, opts, count, row_offset, n_rows):
+    """Recompute pass of the adjoint: rows [n_rows, 9 + C] (see include/diver.h)."""
+    dev = _dev(grid.voxels, grid.mask, o, d, count, row_offset)
+    o, d = _f32(o, 'o'), _f32(d, 'd')
+    count = _i32(count, 'count')
+    if row_offset.dtype != torch.int64:
+        raise TypeError('row_offset must be int64')
+    row_offset = row_offset.contiguous()
+    rows = torch.empty((int(n_rows), 9 + grid.channels), dtype=torch.float32, device=dev)
+    g = _grid_of(grid)
+    _lib.call('diver_render_rows', ctypes.byref(g), decoder.handle, ctypes.byref(opts),
+              o.shape[0], o.data_ptr(), d.data_ptr(), count.data_ptr(), row_offset.data_ptr(),
+              rows.data_ptr(), _s(o))
+    return rows
+
+
+def decoder_update(decoder, params):
+    """Copy canonical parameters (device float32 tensor) into the decoder handle."""
+    _dev(params)
+    p = _f32(params.detach(), 'params')
+    _lib.call('diver_decoder_update', decoder.handle, p.data_ptr(), p.numel(), _s(p))
+
+
 def render_rays(grid, decoder, o, d, opts, with_count=True):
     """grid: an object with voxels, mask, octrees, xyzmin, voxel_size (renderer.SceneGrid)."""
     dev = _dev(grid.voxels, grid.mask, o, d)

@@ -1,0 +1,86 @@
+// Recompute pass of the render adjoint.
+//
+// The forward fused kernel keeps no per-interval samples: only (rgb, T) and the number of
+// intervals composited per ray.  The backward re-walks each ray (same walk, same FP32 integral,
+// same compensated transmittance, so the same values) and writes one row per composited
+// interval at the ray's row offset:
+//   [p_in (3), p_out (3), s = w_s.F + b_s, T_before, alpha, F (C)]
+// The decoder and compositing adjoints then run as batched GEMMs over the rows, and the grid
+// gradient is scattered by the segment-integral adjoint.
+#include "common.h"
+#include "decoder.h"
+#include "kernels.h"
+#include "raystate.h"
+
+namespace diver {
+
+template <class D, bool RT>
+__global__ void __launch_bounds__(256) rows_kernel(RenderArgs a, const int32_t* __restrict__ count,
+                                                   const int64_t* __restrict__ row_off,
+                                                   float* __restrict__ rows) {
+    static_assert(D::NPRE == 0, "rows are written for decoders without pre-layers");
+    constexpr int C = D::C;
+    constexpr int RW = 9 + C;
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= a.n_rays) return;
+    const int n = a.n;
+    RayWalk ray;
+    if constexpr (RT)
+        ray.init_rt(a.o[3 * r], a.o[3 * r + 1], a.o[3 * r + 2], a.d[3 * r], a.d[3 * r + 1],
+                    a.d[3 * r + 2], a.xyzmin, a.voxel_size, n, a.octrees, true);
+    else
+        ray.init(a.o[3 * r], a.o[3 * r + 1], a.o[3 * r + 2], a.d[3 * r], a.d[3 * r + 1],
+                 a.d[3 * r + 2], a.xyzmin, a.voxel_size, (float)n, true);
+    const int todo = count[r];
+    float* out = rows + row_off[r] * RW;
+    Kahan S;
+    float T = 1.0f;
+    for (int k = 0; k < todo; ++k) {
+        float px, py, pz, qx, qy, qz;
+        if (!ray.next<RT>(a.mask, n, px, py, pz, qx, qy, qz)) break;
+        float w[8];
+        int64_t off[8];
+        segment_corners<C>(px, py, pz, qx, qy, qz, n, w, off);
+        float f[C];
+        gather_integral<C>(a.voxels, w, off, f);
+        float s = a.W[D::bs_off];
+#pragma unroll
+        for (int i = 0; i < C; ++i) s += a.W[D::ws_off + i] * f[i];
+        float sigma = density_act<D>(s);
+        float alpha = -expm1f(-sigma);
+        float* o = out + (int64_t)k * RW;
+        o[0] = px; o[1] = py; o[2] = pz;
+        o[3] = qx; o[4] = qy; o[5] = qz;
+        o[6] = s;
+        o[7] = T;
+        o[8] = alpha;
+#pragma unroll
+        for (int i = 0; i < C; ++i) o[9 + i] = f[i];
+        S.add(sigma);
+        T = __expf(-S.s);
+    }
+}
+
+template <class D>
+static hipError_t launch_rows_d(const RenderArgs& a, const int32_t* count, const int64_t* off,
+                                float* rows, hipStream_t s) {
+    dim3 grid((unsigned)((a.n_rays + 255) / 256)), block(256);
+    if (a.traversal == 1)
+        hipLaunchKernelGGL((rows_kernel<D, true>), grid, block, 0, s, a, count, off, rows);
+    else
+        hipLaunchKernelGGL((rows_kernel<D, false>), grid, block, 0, s, a, count, off, rows);
+    return hipGetLastError();
+}
+
+hipError_t launch_rows(int variant, const RenderArgs& a, const int32_t* count, const int64_t* off,
+                       float* rows, hipStream_t s) {
+    if (a.n_rays == 0) return hipSuccess;
+    switch (variant) {
+        case 0: return launch_rows_d<DecB32>(a, count, off, rows, s);
+        case 1: return launch_rows_d<DecB64>(a, count, off, rows, s);
+        case 2: return launch_rows_d<DecB64C64>(a, count, off, rows, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace diver

@@ -74,6 +74,7 @@ struct diver_decoder {
     int device;
     int64_t n_params;
     bool density_ch0 = false;      // sigma = act(F[0]): w_s = e_0, b_s = 0, no pre-layers
+    bool packs_valid = true;       // matrix-core packs match W (false after a device update)
     float* W = nullptr;            // canonical fp32
     uint32_t* pack_bf16 = nullptr; // matrix-core A operands
     uint32_t* pack_f16 = nullptr;
@@ -273,6 +274,18 @@ int diver_decoder_create(const diver_decoder_desc* desc, const float* params_hos
     return DIVER_OK;
 }
 
+int diver_decoder_update(diver_decoder* d, const float* params, int64_t n_params,
+                         diver_stream_t stream) {
+    REQUIRE(d && params, "null argument");
+    REQUIRE(n_params == d->n_params, "decoder expects %lld parameters, got %lld",
+            (long long)d->n_params, (long long)n_params);
+    d->packs_valid = false;
+    d->density_ch0 = false;  // w_s / b_s may have changed: density from the full integral
+    return hip_status(hipMemcpyAsync(d->W, params, sizeof(float) * n_params,
+                                     hipMemcpyDeviceToDevice, S(stream)),
+                      "decoder_update");
+}
+
 int diver_decoder_destroy(diver_decoder* d) {
     if (!d) return DIVER_OK;
     int dev = 0;
@@ -359,6 +372,9 @@ static int render_common(const diver_grid* g, const diver_decoder* dec,
     if (precision != DIVER_FP32 && dec->desc.n_pre != 0)
         return fail(DIVER_EUNSUPPORTED,
                     "the matrix-core decoder path supports decoders without pre-layers only");
+    REQUIRE(precision == DIVER_FP32 || dec->packs_valid,
+            "the decoder was updated from device memory; its BF16/FP16 operands are stale "
+            "(re-create it from host parameters)");
     a.voxels = g->voxels;
     a.mask = g->mask;
     a.octrees = g->octrees;
@@ -413,6 +429,25 @@ int diver_render_camera(const diver_grid* grid, const diver_decoder* dec,
     a.count = count;
     return hip_status(dispatch_render(dec, a, opts->precision, S(stream)),
                       "render_camera");
+}
+
+int diver_render_rows(const diver_grid* grid, const diver_decoder* dec,
+                      const diver_render_opts* opts, int64_t n_rays, const float* o,
+                      const float* d, const int32_t* count, const int64_t* row_offset,
+                      float* rows, diver_stream_t stream) {
+    diver::RenderArgs a{};
+    int st = render_common(grid, dec, opts, a);
+    if (st != DIVER_OK) return st;
+    if (dec->desc.n_pre != 0)
+        return fail(DIVER_EUNSUPPORTED, "rows are written for decoders without pre-layers");
+    REQUIRE(opts->alpha_skip == 0.0f, "the adjoint needs alpha_skip == 0");
+    REQUIRE(n_rays >= 0, "n_rays < 0");
+    REQUIRE(n_rays == 0 || (o && d && count && row_offset && rows), "null pointer");
+    a.n_rays = n_rays;
+    a.o = o;
+    a.d = d;
+    return hip_status(diver::launch_rows(dec->variant, a, count, row_offset, rows, S(stream)),
+                      "render_rows");
 }
 
 int diver_camera_rays(const float* c2w_host, int width, int height, float focal, float* o,

@@ -66,6 +66,8 @@ hipError_t launch_render_queue(int variant, const RenderArgs& a, int prec, bool 
                                hipStream_t s);
 hipError_t launch_block_bits(const uint8_t* mask, int n, uint32_t* bits, hipStream_t s);
 hipError_t launch_walk(const RenderArgs& a, int max_n, float* isect, int32_t* num, hipStream_t s);
+hipError_t launch_rows(int variant, const RenderArgs& a, const int32_t* count, const int64_t* off,
+                       float* rows, hipStream_t s);
 hipError_t launch_camera_rays(const CameraDesc& cam, float* o, float* d, hipStream_t s);
 
 }  // namespace diver

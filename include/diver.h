@@ -130,6 +130,11 @@ typedef struct diver_decoder diver_decoder;
 int diver_decoder_create(const diver_decoder_desc* desc, const float* params_host,
                          int64_t n_params, diver_decoder** out);
 int diver_decoder_destroy(diver_decoder* dec);
+/* Replace the parameters from device memory (asynchronous copy on `stream`, e.g. after an
+ * optimizer step).  Only the FP32 decoder path may be used afterwards (the BF16/FP16 operands
+ * are derived at creation). */
+int diver_decoder_update(diver_decoder* dec, const float* params, int64_t n_params,
+                         diver_stream_t stream);
 int64_t diver_decoder_param_count(const diver_decoder_desc* desc);
 
 /* Real-time decode + composite of one march round.  Replaces mlp_eval(rgba, coord, voxels, v,
@@ -187,6 +192,20 @@ int diver_render_camera(const diver_grid* grid, const diver_decoder* dec,
                         const diver_render_opts* opts, const float* c2w_host, int width,
                         int height, float focal, float* rgba, int32_t* count,
                         diver_stream_t stream);
+
+/* Recompute pass of the render adjoint (decoders without pre-layers, alpha_skip == 0).  Given
+ * the forward's per-ray interval counts and their exclusive prefix sum row_offset [n_rays]
+ * (int64), re-walks every ray and writes, for each composited interval k of ray r, row
+ * row_offset[r] + k of rows [sum(count)][9 + channels]:
+ *   p_in (3), p_out (3), s = w_s.F + b_s, transmittance before the interval, alpha, F (channels)
+ * with the values the forward used.  This replaces the samples the reference's autograd keeps
+ * through integrate / mlp / render (model/diver.py:141-207, utils/integrator.py:62-113); the
+ * decoder and compositing adjoints run as GEMMs over the rows and diver_integrate_backward
+ * scatters the feature gradient into the grid. */
+int diver_render_rows(const diver_grid* grid, const diver_decoder* dec,
+                      const diver_render_opts* opts, int64_t n_rays, const float* o,
+                      const float* d, const int32_t* count, const int64_t* row_offset,
+                      float* rows, diver_stream_t stream);
 
 /* 8^3-block occupancy bits of an [n^3] mask (n a multiple of 8): bit b of word b/32 is set
  * when block b = (bz*(n/8) + by)*(n/8) + bx holds an occupied voxel; bits has
