@@ -35,6 +35,8 @@ def main():
     ap.add_argument('--plain-frames', type=int, default=5)
     ap.add_argument('--cpu-rays', type=int, default=16384)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--calib-copy-mb', type=int, default=0,
+                    help='copy this many MiB device-to-device first (counter calibration)')
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({'error': 'this build runs on one GPU; multi-GPU sharding is described '
@@ -60,6 +62,12 @@ def main():
     dec = Decoder(from_baseline(layers, 32))
     cams = S.hemisphere_cameras()
     t_stop = 1e-4
+    if args.calib_copy_mb:
+        # a plain copy kernel of known size: its FETCH_SIZE / WRITE_SIZE scale the counters
+        src = torch.ones(args.calib_copy_mb * 262144, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        del src, dst
     torch.cuda.synchronize()
 
     def timed(fn, frames):
