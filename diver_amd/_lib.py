@@ -19,7 +19,7 @@ EXPORTS = [
     'diver_composite', 'diver_composite_backward', 'diver_decoder_create', 'diver_decoder_destroy',
     'diver_decoder_param_count', 'diver_mlp_eval', 'diver_render_rays', 'diver_render_camera',
     'diver_camera_rays', 'diver_block_bits', 'diver_walk_rays', 'diver_render_rows',
-    'diver_decoder_update',
+    'diver_decoder_update', 'diver_brick_bits',
 ]
 
 
@@ -41,7 +41,8 @@ class DecoderDesc(ctypes.Structure):
 
 class Grid(ctypes.Structure):
     _fields_ = [('voxels', ctypes.c_void_p), ('mask', ctypes.c_void_p),
-                ('octrees', ctypes.c_void_p), ('blocks', ctypes.c_void_p), ('n', ctypes.c_int),
+                ('octrees', ctypes.c_void_p), ('blocks', ctypes.c_void_p),
+                ('bricks', ctypes.c_void_p), ('n', ctypes.c_int),
                 ('channels', ctypes.c_int), ('xyzmin', ctypes.c_float),
                 ('voxel_size', ctypes.c_float)]
 
@@ -77,6 +78,7 @@ def _declare(L):
                                 ctypes.POINTER(ctypes.c_float), i32, i32, fl, vp, vp, vp],
         'diver_camera_rays': [ctypes.POINTER(ctypes.c_float), i32, i32, fl, vp, vp, vp],
         'diver_block_bits': [vp, i32, vp, vp],
+        'diver_brick_bits': [vp, i32, vp, vp],
         'diver_walk_rays': [ctypes.POINTER(Grid), i32, i64, vp, vp, i32, vp, vp, vp],
         'diver_render_rows': [ctypes.POINTER(Grid), vp, ctypes.POINTER(RenderOpts), i64, vp, vp,
                               vp, vp, vp, vp],

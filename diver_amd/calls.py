@@ -175,12 +175,21 @@ PRECISIONS = {'fp32': 0, 'bf16': 1, 'fp16': 2}
 TRAVERSALS = {'offline': 0, 'realtime': 1}
 
 
-def grid_struct(voxels, mask, octrees, xyzmin, voxel_size, blocks=None):
+def grid_struct(voxels, mask, octrees, xyzmin, voxel_size, blocks=None, bricks=None):
     n = mask.shape[0]
-    return _lib.Grid(voxels.data_ptr(), mask.data_ptr(),
-                     octrees.data_ptr() if octrees is not None else None,
-                     blocks.data_ptr() if blocks is not None else None, int(n),
+    p = lambda t: t.data_ptr() if t is not None else None
+    return _lib.Grid(voxels.data_ptr(), mask.data_ptr(), p(octrees), p(blocks), p(bricks), int(n),
                      int(voxels.shape[-1]), float(xyzmin), float(voxel_size))
+
+
+def brick_bits(mask):
+    """Bit-packed mask, one int64 word per 4^3 brick (see include/diver.h)."""
+    _dev(mask)
+    mask = _u8(mask, 'mask')
+    n = mask.shape[0]
+    bricks = torch.empty((n // 4) ** 3, dtype=torch.int64, device=mask.device)
+    _lib.call('diver_brick_bits', mask.data_ptr(), int(n), bricks.data_ptr(), _s(mask))
+    return bricks
 
 
 def block_bits(mask):
@@ -197,7 +206,7 @@ def block_bits(mask):
 
 def _grid_of(grid):
     return grid_struct(grid.voxels, grid.mask, grid.octrees, grid.xyzmin, grid.voxel_size,
-                       getattr(grid, 'blocks', None))
+                       getattr(grid, 'blocks', None), getattr(grid, 'bricks', None))
 
 
 def render_opts(precision='fp32', traversal='realtime', t_stop=1e-4, alpha_skip=0.0):

@@ -316,7 +316,7 @@ int diver_mlp_eval(const diver_decoder* dec, int64_t batch, const float* coord,
 // when the block bitmap is available; everything else uses the round kernel.
 static hipError_t dispatch_render(const diver_decoder* dec, const diver::RenderArgs& a,
                                   int precision, hipStream_t s) {
-    if (dec->desc.n_pre == 0 && a.blocks)
+    if (dec->desc.n_pre == 0 && a.blocks && a.bricks)
         return diver::launch_render_queue(dec->variant, a, precision, dec->density_ch0, s);
     return diver::launch_render(dec->variant, a, precision, s);
 }
@@ -330,11 +330,13 @@ int diver_walk_rays(const diver_grid* g, int traversal, int64_t n_rays, const fl
             "bad traversal %d", traversal);
     if (traversal == DIVER_WALK_REALTIME)
         REQUIRE(g->octrees && g->n % 64 == 0, "the real-time walk needs the pyramid, n % 64 == 0");
-    REQUIRE(g->blocks && g->n % 8 == 0, "the walk needs the block bits (n % 8 == 0)");
+    REQUIRE(g->blocks && g->bricks && g->n % 8 == 0,
+            "the walk needs the block and brick bits (n % 8 == 0)");
     diver::RenderArgs a{};
     a.mask = g->mask;
     a.octrees = g->octrees;
     a.blocks = g->blocks;
+    a.bricks = reinterpret_cast<const unsigned long long*>(g->bricks);
     a.traversal = traversal;
     a.n = g->n;
     a.xyzmin = g->xyzmin;
@@ -343,6 +345,14 @@ int diver_walk_rays(const diver_grid* g, int traversal, int64_t n_rays, const fl
     a.o = o;
     a.d = d;
     return hip_status(diver::launch_walk(a, max_n, isect, num, S(stream)), "walk_rays");
+}
+
+int diver_brick_bits(const uint8_t* mask, int n, uint64_t* bricks, diver_stream_t stream) {
+    REQUIRE(mask && bricks, "null pointer");
+    REQUIRE(n >= 4 && n % 4 == 0, "n must be a positive multiple of 4");
+    return hip_status(diver::launch_brick_bits(mask, n, reinterpret_cast<unsigned long long*>(bricks),
+                                               S(stream)),
+                      "brick_bits");
 }
 
 int diver_block_bits(const uint8_t* mask, int n, uint32_t* bits, diver_stream_t stream) {
@@ -379,7 +389,9 @@ static int render_common(const diver_grid* g, const diver_decoder* dec,
     a.mask = g->mask;
     a.octrees = g->octrees;
     a.blocks = g->blocks;
-    if (g->blocks) REQUIRE(g->n % 8 == 0, "block bits need n to be a multiple of 8");
+    a.bricks = reinterpret_cast<const unsigned long long*>(g->bricks);
+    if (g->blocks || g->bricks)
+        REQUIRE(g->n % 8 == 0, "block and brick bits need n to be a multiple of 8");
     a.traversal = opts->traversal;
     a.n = g->n;
     a.channels = g->channels;

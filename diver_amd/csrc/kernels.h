@@ -17,6 +17,7 @@ struct RenderArgs {
     const uint8_t* mask;   // [N^3]
     const uint8_t* octrees;  // occupancy pyramid (block sizes 64, 16, 4), real-time traversal
     const uint32_t* blocks;  // 8^3-block occupancy bits (optional)
+    const unsigned long long* bricks;  // bit-packed mask, 64 bits per 4^3 brick (optional)
     int traversal;         // 0: masked_intersect walk (exit > N); 1: real-time walk
     int n;
     int channels;
@@ -65,6 +66,8 @@ hipError_t launch_render(int variant, const RenderArgs& a, int prec, hipStream_t
 hipError_t launch_render_queue(int variant, const RenderArgs& a, int prec, bool dch0,
                                hipStream_t s);
 hipError_t launch_block_bits(const uint8_t* mask, int n, uint32_t* bits, hipStream_t s);
+hipError_t launch_brick_bits(const uint8_t* mask, int n, unsigned long long* bricks,
+                             hipStream_t s);
 hipError_t launch_walk(const RenderArgs& a, int max_n, float* isect, int32_t* num, hipStream_t s);
 hipError_t launch_rows(int variant, const RenderArgs& a, const int32_t* count, const int64_t* off,
                        float* rows, hipStream_t s);

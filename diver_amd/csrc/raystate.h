@@ -64,9 +64,12 @@ struct RayWalk {
     // occupancy pyramid (levels N/64, N/16, N/4, coarse first), descending x4 at each occupied
     // cell; the walk starts at the fine-level entry of the first occupied N/4 cell, or the ray is
     // dead.  Bit-identical to aabb_intersect's start point.
+    // `blocks` (optional, 8^3-block occupancy bits) lets empty blocks skip the octree read at the
+    // finest level (a 4^3 cell inside an empty 8^3 block is empty).
     __device__ __forceinline__ void init_rt(float ox, float oy, float oz, float dx_, float dy_,
                                             float dz_, float xyzmin, float voxel_size, int n,
-                                            const uint8_t* __restrict__ octrees, bool valid) {
+                                            const uint8_t* __restrict__ octrees, bool valid,
+                                            const uint32_t* blocks = nullptr) {
 #pragma clang fp contract(off)
         const float nf = (float)n;
         set_dir(dx_, dy_, dz_);
@@ -101,7 +104,15 @@ struct RayWalk {
             int64_t m3 = (int64_t)m * m * m;
             int64_t ci = ((int64_t)vz * m + vy) * m + vx;
             int64_t gi = off + ci;
-            bool occ = (ci >= 0 && ci < m3 && gi < total) ? octrees[gi] != 0 : false;
+            bool occ;
+            if (blocks && m * 4 == n && (unsigned)vx < (unsigned)m && (unsigned)vy < (unsigned)m &&
+                (unsigned)vz < (unsigned)m) {
+                const int nb = n >> 3;
+                int bid = ((vz >> 1) * nb + (vy >> 1)) * nb + (vx >> 1);
+                occ = ((blocks[bid >> 5] >> (bid & 31)) & 1u) ? octrees[gi] != 0 : false;
+            } else {
+                occ = (ci >= 0 && ci < m3 && gi < total) ? octrees[gi] != 0 : false;
+            }
             if (occ) {
                 off += m3;
                 qx *= 4.0f;
@@ -145,6 +156,45 @@ struct RayWalk {
         }
         alive = false;
         return false;
+    }
+};
+
+// Occupancy lookups of the fine walk through a per-lane cache of one 4^3 brick (64 bits of the
+// bit-packed mask): the 8^3-block bitmap (LDS) answers empty blocks, and the brick word is read
+// from memory only when the walk enters a new brick of an occupied block.  Gives exactly
+// mask[ci] for every voxel inside the grid; indices outside it (which the reference wraps) read
+// the byte mask as the reference does.
+struct OccCache {
+    int brick = -1;
+    uint32_t lo = 0, hi = 0;
+
+    __device__ __forceinline__ bool lookup(const uint8_t* __restrict__ mask,
+                                           const uint32_t* blocks,
+                                           const unsigned long long* __restrict__ bricks, int n,
+                                           int vx, int vy, int vz) {
+        if ((unsigned)vx < (unsigned)n && (unsigned)vy < (unsigned)n && (unsigned)vz < (unsigned)n) {
+            const int n4 = n >> 2;
+            int br = ((vz >> 2) * n4 + (vy >> 2)) * n4 + (vx >> 2);
+            if (br != brick) {
+                brick = br;
+                const int nb = n >> 3;
+                int bid = ((vz >> 3) * nb + (vy >> 3)) * nb + (vx >> 3);
+                if ((blocks[bid >> 5] >> (bid & 31)) & 1u) {
+                    unsigned long long w = bricks[br];
+                    lo = (uint32_t)w;
+                    hi = (uint32_t)(w >> 32);
+                } else {
+                    lo = 0u;
+                    hi = 0u;
+                }
+            }
+            int bit = ((vz & 3) << 4) | ((vy & 3) << 2) | (vx & 3);
+            uint32_t word = bit < 32 ? lo : hi;
+            return (word >> (bit & 31)) & 1u;
+        }
+        const int64_t ntot = (int64_t)n * n * n;
+        int64_t ci = ((int64_t)vz * n + vy) * n + vx;
+        return (ci >= 0 && ci < ntot) ? mask[ci] != 0 : false;
     }
 };
 

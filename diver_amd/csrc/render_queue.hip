@@ -24,24 +24,12 @@
 
 namespace diver {
 
-constexpr int kQueue = 128;      // rows per wave queue (phase A can add up to 63 past 64)
+constexpr int kQueue = 96;       // rows per wave queue (phase A stops at >= 32, adds <= 64)
+constexpr int kFill = 32;        // phase A hands over to phase B at this many queued rows
 constexpr int kRowFloats = 8;    // p_in (3), p_out (3), weight, owner (as int bits)
 constexpr int kMaxLdsBlockWords = 8192;
 
 __host__ __device__ constexpr int wave_lds_floats(int H) { return kQueue * kRowFloats + 64 * 3 + 64 * H; }
-
-// occupancy of voxel (vx,vy,vz) through the block bitmap, then the fine mask
-__device__ __forceinline__ bool voxel_occupied(const uint8_t* __restrict__ mask,
-                                               const uint32_t* blocks, int n, int nb, int vx,
-                                               int vy, int vz) {
-    const int64_t ntot = (int64_t)n * n * n;
-    int64_t ci = ((int64_t)vz * n + vy) * n + vx;
-    if ((unsigned)vx < (unsigned)n && (unsigned)vy < (unsigned)n && (unsigned)vz < (unsigned)n) {
-        int bid = ((vz >> 3) * nb + (vy >> 3)) * nb + (vx >> 3);
-        if (!((blocks[bid >> 5] >> (bid & 31)) & 1u)) return false;
-    }
-    return (ci >= 0 && ci < ntot) ? mask[ci] != 0 : false;
-}
 
 template <class D, int PREC>
 __device__ __forceinline__ void shade_tile(const RenderArgs& a, const float* queue,
@@ -105,7 +93,7 @@ __device__ __forceinline__ void shade_tile(const RenderArgs& a, const float* que
 }
 
 template <class D, int PREC, bool RT, bool DCH0>
-__global__ void __launch_bounds__(256) render_queue_kernel(RenderArgs a) {
+__global__ void __launch_bounds__(256, 3) render_queue_kernel(RenderArgs a) {
     static_assert(D::NPRE == 0, "queue kernel needs density from the integrated feature");
     constexpr int C = D::C, H = D::H, Z = D::Z;
     extern __shared__ uint32_t smem[];
@@ -154,9 +142,10 @@ __global__ void __launch_bounds__(256) render_queue_kernel(RenderArgs a) {
     }
     RayWalk ray;
     if constexpr (RT)
-        ray.init_rt(ox, oy, oz, dx, dy, dz, a.xyzmin, a.voxel_size, n, a.octrees, valid);
+        ray.init_rt(ox, oy, oz, dx, dy, dz, a.xyzmin, a.voxel_size, n, a.octrees, valid, blocks);
     else
         ray.init(ox, oy, oz, dx, dy, dz, a.xyzmin, a.voxel_size, (float)n, valid);
+    OccCache occ;
 
     Kahan S;
     float T = 1.0f;
@@ -166,7 +155,7 @@ __global__ void __launch_bounds__(256) render_queue_kernel(RenderArgs a) {
 
     while (true) {
         // ---------------- phase A: walk, density, transmittance, enqueue ------------------------
-        while (qn < 64) {
+        while (qn < kFill) {
             if (!wave_any(ray.alive)) break;
             bool push = false;
             float px = 0.f, py = 0.f, pz = 0.f, qx = 0.f, qy = 0.f, qz = 0.f, wgt = 0.f;
@@ -185,7 +174,7 @@ __global__ void __launch_bounds__(256) render_queue_kernel(RenderArgs a) {
                         segment_voxel1(ray.lx, ray.ly, ray.lz, nx, ny, nz, vx, vy, vz);
                         px = ray.lx; py = ray.ly; pz = ray.lz;
                         ray.lx = nx; ray.ly = ny; ray.lz = nz;
-                        if (voxel_occupied(a.mask, blocks, n, nb, vx, vy, vz)) {
+                        if (occ.lookup(a.mask, blocks, a.bricks, n, vx, vy, vz)) {
                             qx = nx; qy = ny; qz = nz;
                             float w[8];
                             int64_t off[8];
@@ -345,15 +334,15 @@ __global__ void __launch_bounds__(256) walk_kernel(RenderArgs a, int max_n, floa
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= a.n_rays) return;
     const int n = a.n;
-    const int nb = n >> 3;
     RayWalk ray;
     if constexpr (RT)
         ray.init_rt(a.o[3 * r], a.o[3 * r + 1], a.o[3 * r + 2], a.d[3 * r], a.d[3 * r + 1],
-                    a.d[3 * r + 2], a.xyzmin, a.voxel_size, n, a.octrees, true);
+                    a.d[3 * r + 2], a.xyzmin, a.voxel_size, n, a.octrees, true, a.blocks);
     else
         ray.init(a.o[3 * r], a.o[3 * r + 1], a.o[3 * r + 2], a.d[3 * r], a.d[3 * r + 1],
                  a.d[3 * r + 2], a.xyzmin, a.voxel_size, (float)n, true);
     const float nf = (float)n;
+    OccCache occ;
     int count = 0;
     while (ray.alive && ray.steps < ray.max_steps) {
         ++ray.steps;
@@ -362,7 +351,7 @@ __global__ void __launch_bounds__(256) walk_kernel(RenderArgs a, int max_n, floa
         if (RT ? outside_open(nx, ny, nz, nf) : outside(nx, ny, nz, nf)) break;
         int vx, vy, vz;
         segment_voxel1(ray.lx, ray.ly, ray.lz, nx, ny, nz, vx, vy, vz);
-        if (voxel_occupied(a.mask, a.blocks, n, nb, vx, vy, vz)) {
+        if (occ.lookup(a.mask, a.blocks, a.bricks, n, vx, vy, vz)) {
             if (count < max_n) {
                 float* o = isect + (r * max_n + count) * 6;
                 o[0] = ray.lx; o[1] = ray.ly; o[2] = ray.lz;
@@ -405,6 +394,32 @@ __global__ void block_bits_kernel(const uint8_t* __restrict__ mask, int n,
             for (int x = 0; x < 8; ++x) any |= row[x] != 0;
         }
     if (any) atomicOr(&bits[bid >> 5], 1u << (bid & 31));
+}
+
+// 64-bit occupancy word per 4^3 brick: bit (z&3)*16 + (y&3)*4 + (x&3).
+__global__ void brick_bits_kernel(const uint8_t* __restrict__ mask, int n,
+                                  unsigned long long* __restrict__ bricks) {
+    int n4 = n >> 2;
+    int64_t br = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (br >= (int64_t)n4 * n4 * n4) return;
+    int bx = (int)(br % n4), by = (int)((br / n4) % n4), bz = (int)(br / ((int64_t)n4 * n4));
+    unsigned long long w = 0;
+    for (int z = 0; z < 4; ++z)
+        for (int y = 0; y < 4; ++y) {
+            const uint8_t* row = mask + (((int64_t)(bz * 4 + z) * n + (by * 4 + y)) * n + bx * 4);
+            for (int x = 0; x < 4; ++x)
+                if (row[x]) w |= 1ull << ((z << 4) | (y << 2) | x);
+        }
+    bricks[br] = w;
+}
+
+hipError_t launch_brick_bits(const uint8_t* mask, int n, unsigned long long* bricks,
+                             hipStream_t s) {
+    int n4 = n >> 2;
+    int64_t nbr = (int64_t)n4 * n4 * n4;
+    hipLaunchKernelGGL(brick_bits_kernel, dim3((unsigned)((nbr + 255) / 256)), dim3(256), 0, s,
+                       mask, n, bricks);
+    return hipGetLastError();
 }
 
 __global__ void zero_words_kernel(uint32_t* __restrict__ p, int64_t n) {

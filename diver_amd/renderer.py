@@ -31,6 +31,7 @@ class SceneGrid:
         self.octrees = (build_octrees(self.mask.bool()).to(torch.uint8).contiguous()
                         if n % 64 == 0 else None)
         self.blocks = calls.block_bits(self.mask) if n % 8 == 0 else None
+        self.bricks = calls.brick_bits(self.mask) if n % 8 == 0 else None
 
     @classmethod
     def from_synthetic(cls, scene, device) -> 'SceneGrid':

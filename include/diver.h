@@ -157,6 +157,10 @@ typedef struct diver_grid {
                                coarse first, eval_rt.py:236-239); NULL for the offline walk */
     const uint32_t* blocks; /* 8^3-block occupancy bits from diver_block_bits, or NULL; lets
                                empty blocks skip the fine mask (no effect on results) */
+    const uint64_t* bricks; /* bit-packed mask, one 64-bit word per 4^3 brick, from
+                               diver_brick_bits, or NULL.  With blocks and bricks set, decoders
+                               without pre-layers use the queued kernel (walk and density per
+                               lane, colour in 64-row matrix-core batches) */
     int n;
     int channels;
     float xyzmin;
@@ -212,6 +216,11 @@ int diver_render_rows(const diver_grid* grid, const diver_decoder* dec,
  * ceil((n/8)^3 / 32) words.  No counterpart in the reference (an acceleration structure for
  * diver_render_*). */
 int diver_block_bits(const uint8_t* mask, int n, uint32_t* bits, diver_stream_t stream);
+
+/* Bit-packed mask per 4^3 brick (n a multiple of 4): word b = (bz*(n/4) + by)*(n/4) + bx holds
+ * bit (z&3)*16 + (y&3)*4 + (x&3) for voxel (x, y, z) of the brick; bricks has (n/4)^3 words.
+ * No counterpart in the reference (an acceleration structure for diver_render_*). */
+int diver_brick_bits(const uint8_t* mask, int n, uint64_t* bricks, diver_stream_t stream);
 
 /* The occupied intervals the fused renderer composites, one walk per ray (diver_traversal):
  * the offline walk equals masked_intersect (utils/ray_voxel_intersection/src/
