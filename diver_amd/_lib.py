@@ -9,7 +9,7 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, 'libdiver.so')
+LIB_PATH = os.environ.get('DIVER_LIB') or os.path.join(_HERE, 'libdiver.so')
 
 _lib = None
 
@@ -19,8 +19,10 @@ EXPORTS = [
     'diver_composite', 'diver_composite_backward', 'diver_decoder_create', 'diver_decoder_destroy',
     'diver_decoder_param_count', 'diver_mlp_eval', 'diver_render_rays', 'diver_render_camera',
     'diver_camera_rays', 'diver_block_bits', 'diver_walk_rays', 'diver_render_rows',
-    'diver_decoder_update', 'diver_brick_bits',
+    'diver_decoder_update', 'diver_brick_bits', 'diver_render_batch',
 ]
+
+MAX_REQUESTS = 16
 
 
 class DiverUnavailable(RuntimeError):
@@ -50,6 +52,12 @@ class Grid(ctypes.Structure):
 class RenderOpts(ctypes.Structure):
     _fields_ = [('precision', ctypes.c_int), ('traversal', ctypes.c_int),
                 ('t_stop', ctypes.c_float), ('alpha_skip', ctypes.c_float)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [('c2w', ctypes.c_float * 12), ('width', ctypes.c_int), ('height', ctypes.c_int),
+                ('focal', ctypes.c_float), ('n_rays', ctypes.c_int64), ('o', ctypes.c_void_p),
+                ('d', ctypes.c_void_p), ('rgba', ctypes.c_void_p), ('count', ctypes.c_void_p)]
 
 
 def _declare(L):
@@ -82,6 +90,8 @@ def _declare(L):
         'diver_walk_rays': [ctypes.POINTER(Grid), i32, i64, vp, vp, i32, vp, vp, vp],
         'diver_render_rows': [ctypes.POINTER(Grid), vp, ctypes.POINTER(RenderOpts), i64, vp, vp,
                               vp, vp, vp, vp],
+        'diver_render_batch': [ctypes.POINTER(Grid), vp, ctypes.POINTER(RenderOpts),
+                               ctypes.POINTER(Request), i32, vp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

@@ -277,6 +277,46 @@ def render_camera(grid, decoder, c2w, width, height, focal, opts, with_count=Tru
     return rgba, count
 
 
+def render_batch(grid, decoder, requests, opts, with_count=True):
+    """Render several requests in one launch.
+
+    requests: list of dicts, either {'c2w', 'width', 'height', 'focal'} (camera) or
+    {'o', 'd'} (explicit rays [R,3]).  Returns a list of (rgba, count) in request order, rgba
+    [H,W,4] / [R,4] as render_camera / render_rays return them.
+    """
+    dev = _dev(grid.voxels, grid.mask)
+    if len(requests) > _lib.MAX_REQUESTS:
+        raise ValueError(f'at most {_lib.MAX_REQUESTS} requests per launch')
+    table = (_lib.Request * max(1, len(requests)))()
+    outs, keep = [], []
+    for i, rq in enumerate(requests):
+        r = table[i]
+        if 'c2w' in rq:
+            W, H = int(rq['width']), int(rq['height'])
+            rgba = torch.empty((H, W, 4), dtype=torch.float32, device=dev)
+            count = torch.empty((H, W), dtype=torch.int32, device=dev) if with_count else None
+            for k, x in enumerate(list(rq['c2w'].reshape(-1))[:12]):
+                r.c2w[k] = float(x)
+            r.width, r.height, r.focal = W, H, float(rq['focal'])
+        else:
+            _dev(grid.voxels, rq['o'], rq['d'])
+            o, d = _f32(rq['o'], 'o'), _f32(rq['d'], 'd')
+            if o.shape != d.shape or o.dim() != 2 or o.shape[1] != 3:
+                raise ValueError('o and d must both be [R, 3]')
+            R = o.shape[0]
+            rgba = torch.empty((R, 4), dtype=torch.float32, device=dev)
+            count = torch.empty((R,), dtype=torch.int32, device=dev) if with_count else None
+            r.width, r.n_rays, r.o, r.d = 0, R, o.data_ptr(), d.data_ptr()
+            keep += [o, d]
+        r.rgba = rgba.data_ptr()
+        r.count = _lib.ptr(count)
+        outs.append((rgba, count))
+    g = _grid_of(grid)
+    _lib.call('diver_render_batch', ctypes.byref(g), decoder.handle, ctypes.byref(opts), table,
+              len(requests), _s(grid.voxels))
+    return outs
+
+
 def camera_rays(c2w, width, height, focal, device):
     o = torch.empty((height * width, 3), dtype=torch.float32, device=device)
     d = torch.empty((height * width, 3), dtype=torch.float32, device=device)

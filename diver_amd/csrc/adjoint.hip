@@ -46,8 +46,8 @@ __global__ void __launch_bounds__(256) rows_kernel(RenderArgs a, const int32_t* 
         float s = a.W[D::bs_off];
 #pragma unroll
         for (int i = 0; i < C; ++i) s += a.W[D::ws_off + i] * f[i];
-        float sigma = density_act<D>(s);
-        float alpha = -expm1f(-sigma);
+        float sigma, alpha;
+        density_alpha<D>(s, sigma, alpha);
         float* o = out + (int64_t)k * RW;
         o[0] = px; o[1] = py; o[2] = pz;
         o[3] = qx; o[4] = qy; o[5] = qz;

@@ -312,13 +312,35 @@ int diver_mlp_eval(const diver_decoder* dec, int64_t batch, const float* coord,
                       "mlp_eval");
 }
 
-// Decoders without pre-layers use the queue kernel (per-lane walk + density, batched colour)
-// when the block bitmap is available; everything else uses the round kernel.
-static hipError_t dispatch_render(const diver_decoder* dec, const diver::RenderArgs& a,
-                                  int precision, hipStream_t s) {
-    if (dec->desc.n_pre == 0 && a.blocks && a.bricks)
-        return diver::launch_render_queue(dec->variant, a, precision, dec->density_ch0, s);
-    return diver::launch_render(dec->variant, a, precision, s);
+namespace {
+// The tile kernel indexes features with 32-bit offsets and keeps the block bits in LDS.
+bool tiles_ok(const diver_decoder* dec, const diver::RenderArgs& a) {
+    const double corners = (double)(a.n + 1) * (a.n + 1) * (a.n + 1) * a.channels;
+    return dec->desc.n_pre == 0 && dec->variant >= 0 && dec->variant <= 2 && a.blocks && a.bricks &&
+           corners < 4294967296.0 && diver::tile_fits(a.n);
+}
+}  // namespace
+
+// Decoders without pre-layers use the tile kernel (per-lane walk + density, colour in 64-row
+// matrix-core batches) when the grid's block and brick bits are available; everything else uses
+// the round kernel.
+static int dispatch_render(const diver_decoder* dec, const diver::RenderArgs& a, int precision,
+                           hipStream_t s, const char* what) {
+    if (!tiles_ok(dec, a))
+        return hip_status(diver::launch_render(dec->variant, a, precision, s), what);
+    diver::BatchArgs p{};
+    p.g = a;
+    p.n_req = 1;
+    p.req[0].cam = a.cam;
+    p.req[0].ray_start = 0;
+    p.req[0].n_rays = a.n_rays;
+    p.req[0].o = a.o;
+    p.req[0].d = a.d;
+    p.req[0].rgba = a.rgba;
+    p.req[0].count = a.count;
+    p.total = a.n_rays;
+    return hip_status(diver::launch_render_tiles(dec->variant, p, precision, dec->density_ch0, s),
+                      what);
 }
 
 int diver_walk_rays(const diver_grid* g, int traversal, int64_t n_rays, const float* o,
@@ -418,8 +440,7 @@ int diver_render_rays(const diver_grid* grid, const diver_decoder* dec,
     a.cam.width = 0;
     a.rgba = rgba;
     a.count = count;
-    return hip_status(dispatch_render(dec, a, opts->precision, S(stream)),
-                      "render_rays");
+    return dispatch_render(dec, a, opts->precision, S(stream), "render_rays");
 }
 
 int diver_render_camera(const diver_grid* grid, const diver_decoder* dec,
@@ -439,8 +460,55 @@ int diver_render_camera(const diver_grid* grid, const diver_decoder* dec,
     a.n_rays = tiles * 64;
     a.rgba = rgba;
     a.count = count;
-    return hip_status(dispatch_render(dec, a, opts->precision, S(stream)),
-                      "render_camera");
+    return dispatch_render(dec, a, opts->precision, S(stream), "render_camera");
+}
+
+int diver_render_batch(const diver_grid* grid, const diver_decoder* dec,
+                       const diver_render_opts* opts, const diver_request* reqs, int n_req,
+                       diver_stream_t stream) {
+    diver::RenderArgs a{};
+    int st = render_common(grid, dec, opts, a);
+    if (st != DIVER_OK) return st;
+    REQUIRE(n_req >= 0 && n_req <= DIVER_MAX_REQUESTS, "n_req must be in [0, %d]",
+            DIVER_MAX_REQUESTS);
+    REQUIRE(n_req == 0 || reqs, "null request table");
+    if (!tiles_ok(dec, a))
+        return fail(DIVER_EUNSUPPORTED,
+                    "batched rendering needs block and brick bits (n <= 512), a decoder without "
+                    "pre-layers and (n+1)^3 * channels < 2^32");
+    diver::BatchArgs p{};
+    p.g = a;
+    int64_t total = 0;
+    for (int i = 0; i < n_req; ++i) {
+        const diver_request& r = reqs[i];
+        diver::ReqDesc& q = p.req[i];
+        REQUIRE(r.rgba || (r.width == 0 && r.n_rays == 0), "request %d: null rgba", i);
+        if (r.width > 0) {
+            REQUIRE(r.height > 0 && r.focal > 0, "request %d: bad camera", i);
+            std::memcpy(q.cam.c2w, r.c2w, sizeof(q.cam.c2w));
+            q.cam.width = r.width;
+            q.cam.height = r.height;
+            q.cam.focal = r.focal;
+            q.n_rays = (int64_t)((r.width + 7) / 8) * ((r.height + 7) / 8) * 64;
+        } else {
+            REQUIRE(r.width == 0, "request %d: negative width", i);
+            REQUIRE(r.n_rays >= 0, "request %d: n_rays < 0", i);
+            REQUIRE(r.n_rays == 0 || (r.o && r.d), "request %d: null rays", i);
+            q.cam.width = 0;
+            q.n_rays = r.n_rays;
+            q.o = r.o;
+            q.d = r.d;
+        }
+        q.rgba = r.rgba;
+        q.count = r.count;
+        q.ray_start = total;
+        total += q.n_rays;
+    }
+    p.n_req = n_req;
+    p.total = total;
+    return hip_status(diver::launch_render_tiles(dec->variant, p, opts->precision,
+                                                 dec->density_ch0, S(stream)),
+                      "render_batch");
 }
 
 int diver_render_rows(const diver_grid* grid, const diver_decoder* dec,

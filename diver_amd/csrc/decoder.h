@@ -69,6 +69,8 @@ __device__ __forceinline__ float softplusf(float x) {
 __device__ __forceinline__ float sigmoidf(float x) { return 1.0f / (1.0f + __expf(-x)); }
 
 // PE(d) in the reference's torch order: [d, sin(pi d), cos(pi d), sin(2 pi d), ... cos(8 pi d)].
+// sin(2^l pi d) is evaluated as sinpi(2^l d): 2^l d is exact, so only the sinpi rounding remains
+// (the reference rounds pi * 2^l * d first, model/mlps.py:15-16).
 __device__ __forceinline__ void positional_encoding(float dx, float dy, float dz, float* pe) {
     pe[0] = dx;
     pe[1] = dy;
@@ -76,11 +78,10 @@ __device__ __forceinline__ void positional_encoding(float dx, float dy, float dz
     const float d[3] = {dx, dy, dz};
 #pragma unroll
     for (int l = 0; l < 4; ++l) {
-        const float f = 3.14159265358979323846f * (float)(1 << l);
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             float s, c;
-            sincosf(f * d[a], &s, &c);
+            sincospif((float)(1 << l) * d[a], &s, &c);
             pe[3 + 6 * l + a] = s;
             pe[6 + 6 * l + a] = c;
         }
@@ -149,6 +150,22 @@ template <class D>
 __device__ __forceinline__ float density_act(float s) {
     if constexpr (D::DACT == 0) return fmaxf(s, 0.0f);
     else return softplusf(s);
+}
+
+// Density and opacity of one interval from the pre-activation s: sigma = act(s) and
+// alpha = 1 - exp(-sigma).  For softplus, exp(-softplus(s)) = 1 / (1 + e^s), so with
+// e = exp(-|s|): alpha = 1 / (1 + e) for s > 0 and e / (1 + e) otherwise, which shares the
+// exponential with softplus.
+template <class D>
+__device__ __forceinline__ void density_alpha(float s, float& sigma, float& alpha) {
+    if constexpr (D::DACT == 0) {
+        sigma = fmaxf(s, 0.0f);
+        alpha = -expm1f(-sigma);
+    } else {
+        const float e = __expf(-fabsf(s));
+        sigma = fmaxf(s, 0.0f) + log1pf(e);
+        alpha = (s > 0.0f ? 1.0f : e) * __builtin_amdgcn_rcpf(1.0f + e);
+    }
 }
 
 // Colour of one row from the head input z and the hoisted hv (FP32, one lane).

@@ -35,6 +35,28 @@ struct RenderArgs {
     int32_t* count;        // [rays] intervals consumed (optional)
 };
 
+// One request of a batched render: a camera view (width > 0) or explicit rays.  Its rays are
+// [ray_start, ray_start + n_rays) of the launch's ray space (camera: tiles * 64 in tile order).
+struct ReqDesc {
+    CameraDesc cam;
+    int64_t ray_start;
+    int64_t n_rays;
+    const float* o;
+    const float* d;
+    float* rgba;
+    int32_t* count;
+};
+
+constexpr int kMaxRequests = 16;
+
+// A launch over a table of requests (diver_render_batch; a single render is a table of one).
+struct BatchArgs {
+    RenderArgs g;                  // grid, decoder, options (ray fields unused)
+    ReqDesc req[kMaxRequests];
+    int n_req;
+    int64_t total;                 // ray ids over all requests
+};
+
 hipError_t launch_ray_voxel_intersect(int64_t batch, int max_n, float xyzmin, float voxel_num,
                                       float voxel_size, const float* o, const float* v,
                                       float* isect, int32_t* num, float* tns, hipStream_t s);
@@ -63,8 +85,9 @@ hipError_t launch_mlp_eval(int variant, int64_t batch, const float* coord, const
                            uint8_t* finish, const float* W, float alpha_skip, float t_stop,
                            hipStream_t s);
 hipError_t launch_render(int variant, const RenderArgs& a, int prec, hipStream_t s);
-hipError_t launch_render_queue(int variant, const RenderArgs& a, int prec, bool dch0,
+hipError_t launch_render_tiles(int variant, const BatchArgs& p, int prec, bool dch0,
                                hipStream_t s);
+bool tile_fits(int n);  // the grid's block bitmap fits the tile kernel's LDS
 hipError_t launch_block_bits(const uint8_t* mask, int n, uint32_t* bits, hipStream_t s);
 hipError_t launch_brick_bits(const uint8_t* mask, int n, unsigned long long* bricks,
                              hipStream_t s);

@@ -64,12 +64,9 @@ struct RayWalk {
     // occupancy pyramid (levels N/64, N/16, N/4, coarse first), descending x4 at each occupied
     // cell; the walk starts at the fine-level entry of the first occupied N/4 cell, or the ray is
     // dead.  Bit-identical to aabb_intersect's start point.
-    // `blocks` (optional, 8^3-block occupancy bits) lets empty blocks skip the octree read at the
-    // finest level (a 4^3 cell inside an empty 8^3 block is empty).
     __device__ __forceinline__ void init_rt(float ox, float oy, float oz, float dx_, float dy_,
                                             float dz_, float xyzmin, float voxel_size, int n,
-                                            const uint8_t* __restrict__ octrees, bool valid,
-                                            const uint32_t* blocks = nullptr) {
+                                            const uint8_t* __restrict__ octrees, bool valid) {
 #pragma clang fp contract(off)
         const float nf = (float)n;
         set_dir(dx_, dy_, dz_);
@@ -104,15 +101,7 @@ struct RayWalk {
             int64_t m3 = (int64_t)m * m * m;
             int64_t ci = ((int64_t)vz * m + vy) * m + vx;
             int64_t gi = off + ci;
-            bool occ;
-            if (blocks && m * 4 == n && (unsigned)vx < (unsigned)m && (unsigned)vy < (unsigned)m &&
-                (unsigned)vz < (unsigned)m) {
-                const int nb = n >> 3;
-                int bid = ((vz >> 1) * nb + (vy >> 1)) * nb + (vx >> 1);
-                occ = ((blocks[bid >> 5] >> (bid & 31)) & 1u) ? octrees[gi] != 0 : false;
-            } else {
-                occ = (ci >= 0 && ci < m3 && gi < total) ? octrees[gi] != 0 : false;
-            }
+            const bool occ = (ci >= 0 && ci < m3 && gi < total) ? octrees[gi] != 0 : false;
             if (occ) {
                 off += m3;
                 qx *= 4.0f;
@@ -199,10 +188,12 @@ struct OccCache {
 };
 
 // Corner weights of the segment mean of the trilinear field (integrator.py:92-111 arranged as
-// mlp_eval.cu:83-101) and the 8 corner offsets into a [(n+1)^3][C] grid.
-template <int C>
+// mlp_eval.cu:83-101) and the 8 corner offsets into a [(n+1)^3][C] grid.  I is the offset type:
+// uint32_t when the caller has checked (n+1)^3 * C < 2^32 (cheaper index arithmetic), else
+// int64_t.
+template <int C, typename I = int64_t>
 __device__ __forceinline__ void segment_corners(float px, float py, float pz, float qx, float qy,
-                                                float qz, int n, float (&w)[8], int64_t (&off)[8]) {
+                                                float qz, int n, float (&w)[8], I (&off)[8]) {
     int cx, cy, cz;
     {
 #pragma clang fp contract(off)
@@ -229,19 +220,19 @@ __device__ __forceinline__ void segment_corners(float px, float py, float pz, fl
     w[1] = x01 - w[5] - w4;
     w[3] = w4 - w[7];
     const int cx1 = min(cx + 1, n), cy1 = min(cy + 1, n), cz1 = min(cz + 1, n);
-    const int64_t sz = (int64_t)(n + 1) * (n + 1) * C, sy = (int64_t)(n + 1) * C;
+    const I sz = (I)(n + 1) * (I)(n + 1) * (I)C, sy = (I)(n + 1) * (I)C;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
         int gx = (j & 1) ? cx1 : cx;
         int gy = ((j >> 1) & 1) ? cy1 : cy;
         int gz = (j >> 2) ? cz1 : cz;
-        off[j] = gz * sz + gy * sy + (int64_t)gx * C;
+        off[j] = (I)gz * sz + (I)gy * sy + (I)gx * (I)C;
     }
 }
 
-template <int C>
+template <int C, typename I>
 __device__ __forceinline__ void gather_integral(const float* __restrict__ voxels, const float (&w)[8],
-                                                const int64_t (&off)[8], float (&f)[C]) {
+                                                const I (&off)[8], float (&f)[C]) {
 #pragma unroll
     for (int i = 0; i < C; ++i) f[i] = 0.0f;
 #pragma unroll

@@ -96,14 +96,14 @@ __global__ void __launch_bounds__(256) render_kernel(RenderArgs a) {
 
         // ---- segment integral (FP32) ----------------------------------------------------------
         float f[C];
-        float sigma = 0.0f;
+        float sigma = 0.0f, alpha = 0.0f;
         float z[Z];
         if (has) {
             float w[8];
             int64_t off[8];
             segment_corners<C>(px, py, pz, qx, qy, qz, a.n, w, off);
             gather_integral<C>(a.voxels, w, off, f);
-            sigma = density_act<D>(pre_and_sigma<D>(a.W, f, z));
+            density_alpha<D>(pre_and_sigma<D>(a.W, f, z), sigma, alpha);
         } else {
 #pragma unroll
             for (int i = 0; i < Z; ++i) z[i] = 0.0f;
@@ -180,7 +180,6 @@ __global__ void __launch_bounds__(256) render_kernel(RenderArgs a) {
         // ---- composite --------------------------------------------------------------------------
         if (has) {
             used += 1;
-            float alpha = -expm1f(-sigma);
             if (alpha < a.alpha_skip) continue;
             float wgt = T * alpha;
             cr += wgt * rgb[0];

@@ -2,25 +2,31 @@
 // (canonical form with no pre-layers): traversal and density run per lane, colour decoding runs
 // in dense 64-row matrix-core batches.
 //
-// One wavefront owns a packet of 64 rays (one per lane).  It alternates two phases:
+// One wavefront owns a tile of 64 consecutive ray ids of the launch (one ray per lane); the ids
+// run over a table of requests (camera views in 8x8 pixel tiles, or explicit ray batches), so
+// rays of several requests of different sizes share one launch.  A ray's result depends only on
+// its own intervals (its colour sum adds them front to back), never on the rays it shares a
+// wave or a launch with.  The wave alternates two phases:
 //   A. every live lane advances the exact DDA one voxel per iteration (lanes never wait on each
 //      other's hits); at an occupied voxel it integrates only what density needs (channel 0 for
 //      softplus(f[0]) decoders), composites transmittance (compensated log-sum, early stop at
 //      T < t_stop) and pushes the row (entry, exit, weight = T*alpha, owner lane) into the
-//      wave's LDS queue.  The phase ends when >= 64 rows are queued or every lane is done.
+//      wave's LDS queue.  The phase ends when >= 32 rows are queued or every lane is done.
 //   B. the wave takes up to 64 queued rows, one per lane: full FP32 segment integral of the
 //      row's voxel, then the colour decoder on the matrix cores, one 32-row column tile at a
 //      time (or FP32 per lane), and an LDS atomic add of weight * colour into the owner ray's
 //      accumulator.  The per-ray view term W_v PE(d) + b_c is computed once per ray in FP32 and
 //      kept in LDS.
-// Empty-space skipping: the workgroup keeps the 8^3-block occupancy bitmap in LDS; a voxel in an
-// empty block never reads the fine mask.  Skipping changes no arithmetic of the walk, so the
-// intervals are bit-identical to the drop-in traversal kernels.
+// Empty-space skipping: the workgroup keeps the 8^3-block occupancy bitmap in LDS and each lane
+// caches the bit-packed mask of its current 4^3 brick; a voxel in an empty block never reads
+// memory.  Skipping changes no arithmetic of the walk, so the intervals are bit-identical to the
+// drop-in traversal kernels.
 #include "common.h"
 #include "decoder.h"
 #include "kernels.h"
 #include "mfma.h"
 #include "raystate.h"
+#include "requests.h"
 
 namespace diver {
 
@@ -93,19 +99,18 @@ __device__ __forceinline__ void shade_tile(const RenderArgs& a, const float* que
 }
 
 template <class D, int PREC, bool RT, bool DCH0>
-__global__ void __launch_bounds__(256, 3) render_queue_kernel(RenderArgs a) {
-    static_assert(D::NPRE == 0, "queue kernel needs density from the integrated feature");
+__global__ void __launch_bounds__(256, 3) render_tile_kernel(BatchArgs p) {
+    static_assert(D::NPRE == 0, "the tile kernel needs density from the integrated feature");
     constexpr int C = D::C, H = D::H, Z = D::Z;
+    const RenderArgs& a = p.g;
     extern __shared__ uint32_t smem[];
     const int n = a.n;
     const int nb = n >> 3;
     const int nwords = (nb * nb * nb + 31) >> 5;
-    const bool lds_blocks = nwords <= kMaxLdsBlockWords;
-    uint32_t* sblocks = smem;
-    float* wave_mem = reinterpret_cast<float*>(smem + (lds_blocks ? ((nwords + 3) & ~3) : 0));
-    for (int i = threadIdx.x; lds_blocks && i < nwords; i += blockDim.x) sblocks[i] = a.blocks[i];
+    uint32_t* blocks = smem;
+    float* wave_mem = reinterpret_cast<float*>(smem + ((nwords + 3) & ~3));
+    for (int i = threadIdx.x; i < nwords; i += blockDim.x) blocks[i] = a.blocks[i];
     __syncthreads();
-    const uint32_t* blocks = lds_blocks ? sblocks : a.blocks;
 
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
@@ -116,22 +121,15 @@ __global__ void __launch_bounds__(256, 3) render_queue_kernel(RenderArgs a) {
     acc[lane * 3 + 1] = 0.0f;
     acc[lane * 3 + 2] = 0.0f;
 
-    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int64_t r = wave * 64 + lane;
-    bool valid;
-    int64_t out_idx = r;
-    float ox, oy, oz, dx, dy, dz;
-    if (a.cam.width > 0) {
-        int x, y;
-        valid = (r < a.n_rays) && camera_pixel(a.cam, r, x, y);
-        if (!valid) { x = 0; y = 0; }
-        out_idx = (int64_t)y * a.cam.width + x;
-        camera_ray(a.cam, x, y, ox, oy, oz, dx, dy, dz);
-    } else {
-        valid = r < a.n_rays;
-        int64_t rr = valid ? r : 0;
-        ox = a.o[3 * rr]; oy = a.o[3 * rr + 1]; oz = a.o[3 * rr + 2];
-        dx = a.d[3 * rr]; dy = a.d[3 * rr + 1]; dz = a.d[3 * rr + 2];
+    const int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    int req = 0;
+    int64_t out_idx = 0;
+    float ox = 0.f, oy = 0.f, oz = 0.f, dx = 0.f, dy = 0.f, dz = 1.f;
+    const bool valid = g < p.total && ray_of(p, g, req, out_idx, ox, oy, oz, dx, dy, dz);
+    if (!valid) {
+        ox = oy = oz = 0.f;
+        dx = dy = 0.f;
+        dz = 1.f;
     }
     {
         float hv[H];
@@ -142,7 +140,7 @@ __global__ void __launch_bounds__(256, 3) render_queue_kernel(RenderArgs a) {
     }
     RayWalk ray;
     if constexpr (RT)
-        ray.init_rt(ox, oy, oz, dx, dy, dz, a.xyzmin, a.voxel_size, n, a.octrees, valid, blocks);
+        ray.init_rt(ox, oy, oz, dx, dy, dz, a.xyzmin, a.voxel_size, n, a.octrees, valid);
     else
         ray.init(ox, oy, oz, dx, dy, dz, a.xyzmin, a.voxel_size, (float)n, valid);
     OccCache occ;
@@ -177,8 +175,8 @@ __global__ void __launch_bounds__(256, 3) render_queue_kernel(RenderArgs a) {
                         if (occ.lookup(a.mask, blocks, a.bricks, n, vx, vy, vz)) {
                             qx = nx; qy = ny; qz = nz;
                             float w[8];
-                            int64_t off[8];
-                            segment_corners<C>(px, py, pz, qx, qy, qz, n, w, off);
+                            uint32_t off[8];
+                            segment_corners<C, uint32_t>(px, py, pz, qx, qy, qz, n, w, off);
                             float s;
                             if constexpr (DCH0) {
                                 s = 0.0f;
@@ -191,8 +189,8 @@ __global__ void __launch_bounds__(256, 3) render_queue_kernel(RenderArgs a) {
 #pragma unroll
                                 for (int i = 0; i < C; ++i) s += a.W[D::ws_off + i] * f[i];
                             }
-                            float sigma = density_act<D>(s);
-                            float alpha = -expm1f(-sigma);
+                            float sigma, alpha;
+                            density_alpha<D>(s, sigma, alpha);
                             used += 1;
                             if (alpha >= a.alpha_skip) {
                                 wgt = T * alpha;
@@ -227,8 +225,8 @@ __global__ void __launch_bounds__(256, 3) render_queue_kernel(RenderArgs a) {
             float4 r0 = *reinterpret_cast<const float4*>(queue + lane * kRowFloats);
             float4 r1 = *reinterpret_cast<const float4*>(queue + lane * kRowFloats + 4);
             float w[8];
-            int64_t off[8];
-            segment_corners<C>(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, n, w, off);
+            uint32_t off[8];
+            segment_corners<C, uint32_t>(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, n, w, off);
             gather_integral<C>(a.voxels, w, off, z);
         } else {
 #pragma unroll
@@ -276,166 +274,58 @@ __global__ void __launch_bounds__(256, 3) render_queue_kernel(RenderArgs a) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     if (valid) {
-        float4 outv = make_float4(acc[lane * 3], acc[lane * 3 + 1], acc[lane * 3 + 2], T);
-        reinterpret_cast<float4*>(a.rgba)[out_idx] = outv;
-        if (a.count) a.count[out_idx] = used;
+        const ReqDesc& rq = p.req[req];
+        reinterpret_cast<float4*>(rq.rgba)[out_idx] =
+            make_float4(acc[lane * 3], acc[lane * 3 + 1], acc[lane * 3 + 2], T);
+        if (rq.count) rq.count[out_idx] = used;
     }
 }
 
-static size_t render_queue_lds_bytes(int n, int H) {
+static size_t render_tile_lds_bytes(int n, int H) {
     int nb = n >> 3;
     int64_t nwords = ((int64_t)nb * nb * nb + 31) >> 5;
-    size_t b = nwords <= kMaxLdsBlockWords ? (size_t)((nwords + 3) & ~3) * 4 : 0;
-    return b + 4 * (size_t)wave_lds_floats(H) * sizeof(float);
+    return (size_t)((nwords + 3) & ~3) * 4 + 4 * (size_t)wave_lds_floats(H) * sizeof(float);
+}
+
+bool tile_fits(int n) {
+    const int nb = n >> 3;
+    const int64_t nwords = ((int64_t)nb * nb * nb + 31) >> 5;
+    return n % 8 == 0 && nwords <= kMaxLdsBlockWords;
 }
 
 template <class D, int PREC, bool RT>
-static hipError_t launch_q3(const RenderArgs& a, bool dch0, hipStream_t s) {
-    int64_t waves = (a.n_rays + 63) / 64;
-    dim3 grid((unsigned)((waves + 3) / 4)), block(256);
-    size_t lds = render_queue_lds_bytes(a.n, D::H);
+static hipError_t launch_t3(const BatchArgs& p, bool dch0, hipStream_t s) {
+    dim3 grid((unsigned)((p.total + 255) / 256)), block(256);
+    size_t lds = render_tile_lds_bytes(p.g.n, D::H);
     if (dch0)
-        hipLaunchKernelGGL((render_queue_kernel<D, PREC, RT, true>), grid, block, lds, s, a);
+        hipLaunchKernelGGL((render_tile_kernel<D, PREC, RT, true>), grid, block, lds, s, p);
     else
-        hipLaunchKernelGGL((render_queue_kernel<D, PREC, RT, false>), grid, block, lds, s, a);
+        hipLaunchKernelGGL((render_tile_kernel<D, PREC, RT, false>), grid, block, lds, s, p);
     return hipGetLastError();
 }
 
 template <class D, bool RT>
-static hipError_t launch_q2(const RenderArgs& a, int prec, bool dch0, hipStream_t s) {
-    if (prec == kPrecBF16) return launch_q3<D, kPrecBF16, RT>(a, dch0, s);
-    if (prec == kPrecFP16) return launch_q3<D, kPrecFP16, RT>(a, dch0, s);
-    return launch_q3<D, kPrecFP32, RT>(a, dch0, s);
+static hipError_t launch_t2(const BatchArgs& p, int prec, bool dch0, hipStream_t s) {
+    if (prec == kPrecBF16) return launch_t3<D, kPrecBF16, RT>(p, dch0, s);
+    if (prec == kPrecFP16) return launch_t3<D, kPrecFP16, RT>(p, dch0, s);
+    return launch_t3<D, kPrecFP32, RT>(p, dch0, s);
 }
 
 template <class D>
-static hipError_t launch_q1(const RenderArgs& a, int prec, bool dch0, hipStream_t s) {
-    return a.traversal == 1 ? launch_q2<D, true>(a, prec, dch0, s)
-                            : launch_q2<D, false>(a, prec, dch0, s);
+static hipError_t launch_t1(const BatchArgs& p, int prec, bool dch0, hipStream_t s) {
+    return p.g.traversal == 1 ? launch_t2<D, true>(p, prec, dch0, s)
+                              : launch_t2<D, false>(p, prec, dch0, s);
 }
 
-hipError_t launch_render_queue(int variant, const RenderArgs& a, int prec, bool dch0,
+hipError_t launch_render_tiles(int variant, const BatchArgs& p, int prec, bool dch0,
                                hipStream_t s) {
-    if (a.n_rays == 0) return hipSuccess;
+    if (p.total == 0) return hipSuccess;
     switch (variant) {
-        case 0: return launch_q1<DecB32>(a, prec, dch0, s);
-        case 1: return launch_q1<DecB64>(a, prec, dch0, s);
-        case 2: return launch_q1<DecB64C64>(a, prec, dch0, s);
+        case 0: return launch_t1<DecB32>(p, prec, dch0, s);
+        case 1: return launch_t1<DecB64>(p, prec, dch0, s);
+        case 2: return launch_t1<DecB64C64>(p, prec, dch0, s);
         default: return hipErrorInvalidValue;
     }
-}
-
-// The fused kernels' walk on its own: every occupied interval of each ray, in masked_intersect's
-// layout ([rays][max_n][6], count per ray).  Same step, entry and occupancy code as the queue
-// kernel's phase A, so its output is what the fused renderer composites.
-template <bool RT>
-__global__ void __launch_bounds__(256) walk_kernel(RenderArgs a, int max_n, float* __restrict__ isect,
-                                                   int32_t* __restrict__ num) {
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= a.n_rays) return;
-    const int n = a.n;
-    RayWalk ray;
-    if constexpr (RT)
-        ray.init_rt(a.o[3 * r], a.o[3 * r + 1], a.o[3 * r + 2], a.d[3 * r], a.d[3 * r + 1],
-                    a.d[3 * r + 2], a.xyzmin, a.voxel_size, n, a.octrees, true, a.blocks);
-    else
-        ray.init(a.o[3 * r], a.o[3 * r + 1], a.o[3 * r + 2], a.d[3 * r], a.d[3 * r + 1],
-                 a.d[3 * r + 2], a.xyzmin, a.voxel_size, (float)n, true);
-    const float nf = (float)n;
-    OccCache occ;
-    int count = 0;
-    while (ray.alive && ray.steps < ray.max_steps) {
-        ++ray.steps;
-        float nx, ny, nz;
-        ray.step(ray.lx, ray.ly, ray.lz, nx, ny, nz);
-        if (RT ? outside_open(nx, ny, nz, nf) : outside(nx, ny, nz, nf)) break;
-        int vx, vy, vz;
-        segment_voxel1(ray.lx, ray.ly, ray.lz, nx, ny, nz, vx, vy, vz);
-        if (occ.lookup(a.mask, a.blocks, a.bricks, n, vx, vy, vz)) {
-            if (count < max_n) {
-                float* o = isect + (r * max_n + count) * 6;
-                o[0] = ray.lx; o[1] = ray.ly; o[2] = ray.lz;
-                o[3] = nx; o[4] = ny; o[5] = nz;
-            }
-            ++count;
-        }
-        ray.lx = nx;
-        ray.ly = ny;
-        ray.lz = nz;
-    }
-    num[r] = count;
-    for (int k = count; k < max_n; ++k) {
-        float* o = isect + (r * max_n + k) * 6;
-        o[0] = 0.f; o[1] = 0.f; o[2] = 0.f; o[3] = 0.f; o[4] = 0.f; o[5] = 0.f;
-    }
-}
-
-hipError_t launch_walk(const RenderArgs& a, int max_n, float* isect, int32_t* num,
-                       hipStream_t s) {
-    if (a.n_rays == 0) return hipSuccess;
-    dim3 grid((unsigned)((a.n_rays + 255) / 256)), block(256);
-    if (a.traversal == 1)
-        hipLaunchKernelGGL(walk_kernel<true>, grid, block, 0, s, a, max_n, isect, num);
-    else
-        hipLaunchKernelGGL(walk_kernel<false>, grid, block, 0, s, a, max_n, isect, num);
-    return hipGetLastError();
-}
-
-__global__ void block_bits_kernel(const uint8_t* __restrict__ mask, int n,
-                                  uint32_t* __restrict__ bits) {
-    int nb = n >> 3;
-    int64_t bid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (bid >= (int64_t)nb * nb * nb) return;
-    int bx = (int)(bid % nb), by = (int)((bid / nb) % nb), bz = (int)(bid / ((int64_t)nb * nb));
-    bool any = false;
-    for (int z = 0; z < 8 && !any; ++z)
-        for (int y = 0; y < 8 && !any; ++y) {
-            const uint8_t* row = mask + (((int64_t)(bz * 8 + z) * n + (by * 8 + y)) * n + bx * 8);
-            for (int x = 0; x < 8; ++x) any |= row[x] != 0;
-        }
-    if (any) atomicOr(&bits[bid >> 5], 1u << (bid & 31));
-}
-
-// 64-bit occupancy word per 4^3 brick: bit (z&3)*16 + (y&3)*4 + (x&3).
-__global__ void brick_bits_kernel(const uint8_t* __restrict__ mask, int n,
-                                  unsigned long long* __restrict__ bricks) {
-    int n4 = n >> 2;
-    int64_t br = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (br >= (int64_t)n4 * n4 * n4) return;
-    int bx = (int)(br % n4), by = (int)((br / n4) % n4), bz = (int)(br / ((int64_t)n4 * n4));
-    unsigned long long w = 0;
-    for (int z = 0; z < 4; ++z)
-        for (int y = 0; y < 4; ++y) {
-            const uint8_t* row = mask + (((int64_t)(bz * 4 + z) * n + (by * 4 + y)) * n + bx * 4);
-            for (int x = 0; x < 4; ++x)
-                if (row[x]) w |= 1ull << ((z << 4) | (y << 2) | x);
-        }
-    bricks[br] = w;
-}
-
-hipError_t launch_brick_bits(const uint8_t* mask, int n, unsigned long long* bricks,
-                             hipStream_t s) {
-    int n4 = n >> 2;
-    int64_t nbr = (int64_t)n4 * n4 * n4;
-    hipLaunchKernelGGL(brick_bits_kernel, dim3((unsigned)((nbr + 255) / 256)), dim3(256), 0, s,
-                       mask, n, bricks);
-    return hipGetLastError();
-}
-
-__global__ void zero_words_kernel(uint32_t* __restrict__ p, int64_t n) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0u;
-}
-
-hipError_t launch_block_bits(const uint8_t* mask, int n, uint32_t* bits, hipStream_t s) {
-    int nb = n >> 3;
-    int64_t nblk = (int64_t)nb * nb * nb;
-    int64_t nwords = (nblk + 31) >> 5;
-    hipLaunchKernelGGL(zero_words_kernel, dim3((unsigned)((nwords + 255) / 256)), dim3(256), 0, s,
-                       bits, nwords);
-    hipLaunchKernelGGL(block_bits_kernel, dim3((unsigned)((nblk + 255) / 256)), dim3(256), 0, s,
-                       mask, n, bits);
-    return hipGetLastError();
 }
 
 }  // namespace diver

@@ -159,8 +159,8 @@ typedef struct diver_grid {
                                empty blocks skip the fine mask (no effect on results) */
     const uint64_t* bricks; /* bit-packed mask, one 64-bit word per 4^3 brick, from
                                diver_brick_bits, or NULL.  With blocks and bricks set, decoders
-                               without pre-layers use the queued kernel (walk and density per
-                               lane, colour in 64-row matrix-core batches) */
+                               without pre-layers use the tile kernel (walk and density per
+                               lane, colour in 64-row matrix-core batches; n <= 512) */
     int n;
     int channels;
     float xyzmin;
@@ -196,6 +196,34 @@ int diver_render_camera(const diver_grid* grid, const diver_decoder* dec,
                         const diver_render_opts* opts, const float* c2w_host, int width,
                         int height, float focal, float* rgba, int32_t* count,
                         diver_stream_t stream);
+
+/* One request of diver_render_batch: a pinhole camera (width > 0; c2w row-major 3x4, copied at
+ * the call) or n_rays explicit rays o, d [n_rays][3] (width == 0).  Outputs as
+ * diver_render_camera / diver_render_rays. */
+typedef struct diver_request {
+    float c2w[12];
+    int width;
+    int height;
+    float focal;
+    int64_t n_rays;
+    const float* o;
+    const float* d;
+    float* rgba;
+    int32_t* count;
+} diver_request;
+
+#define DIVER_MAX_REQUESTS 16
+
+/* Several requests on one grid and decoder in one launch: the rays of all requests are numbered
+ * consecutively (a camera's pixels in 8x8 tiles) and every wave renders 64 consecutive ids, so
+ * requests of different sizes share the launch.  Each request's pixels are computed exactly as a
+ * diver_render_camera / diver_render_rays call of that request alone computes them (a ray's
+ * result depends only on its own intervals).  Needs the block and brick bits and a decoder without
+ * pre-layers.  Up to DIVER_MAX_REQUESTS requests.
+ * No counterpart in the reference (the serving path batches the requests of several clients). */
+int diver_render_batch(const diver_grid* grid, const diver_decoder* dec,
+                       const diver_render_opts* opts, const diver_request* reqs, int n_req,
+                       diver_stream_t stream);
 
 /* Recompute pass of the render adjoint (decoders without pre-layers, alpha_skip == 0).  Given
  * the forward's per-ray interval counts and their exclusive prefix sum row_offset [n_rays]
