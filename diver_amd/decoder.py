@@ -182,6 +182,12 @@ class Decoder:
             pass
 
     @property
+    def nbytes(self) -> int:
+        """Device bytes the handle holds: FP32 parameters plus the BF16 and FP16 matrix-core
+        operand packs (at most the same element count each, at 2 bytes)."""
+        return int(self.canon.params.size) * (4 + 2 + 2)
+
+    @property
     def channels(self):
         return self.canon.channels
 
