@@ -48,7 +48,8 @@ def split_canonical(p: torch.Tensor, C: int, H: int):
 
 class _FusedRender(Function):
     @staticmethod
-    def forward(ctx, voxels, params, grid, decoder, o, d, t_stop, white_back, traversal):
+    def forward(ctx, voxels, params, grid, decoder, o, d, t_stop, white_back, traversal,
+                accumulate):
         calls.decoder_update(decoder, params)
         opts = calls.render_opts('fp32', traversal, t_stop, 0.0)
         rgba, count = calls.render_rays(grid, decoder, o, d, opts)
@@ -56,6 +57,7 @@ class _FusedRender(Function):
         ctx.grid, ctx.decoder, ctx.opts = grid, decoder, opts
         ctx.white_back = white_back
         ctx.vshape = tuple(voxels.shape)
+        ctx.accumulate = voxels if accumulate else None
         bg = rgba[:, 3:] if white_back else torch.zeros_like(rgba[:, 3:])
         return rgba[:, :3] + bg
 
@@ -70,7 +72,8 @@ class _FusedRender(Function):
         K = int(cnt64.sum().item())
         dev = o.device
         if K == 0:
-            return torch.zeros(ctx.vshape, device=dev), torch.zeros_like(params), *([None] * 7)
+            gv0 = None if ctx.accumulate is not None else torch.zeros(ctx.vshape, device=dev)
+            return gv0, torch.zeros_like(params), *([None] * 8)
         calls.decoder_update(decoder, params)
         rows = calls.render_rows(grid, decoder, o, d, ctx.opts, count, row_off, K)
         ray = torch.repeat_interleave(torch.arange(R, device=dev), cnt64)
@@ -85,9 +88,11 @@ class _FusedRender(Function):
         r2 = torch.relu(y2)
         c = torch.sigmoid(r2 @ Wo.T + bo)
         w = Tb * a
-        wc = (w[:, None] * c).double()
-        # suffix sums within each ray (float64 segmented cumsum)
-        cs = torch.cat([torch.zeros(1, 3, dtype=torch.float64, device=dev), torch.cumsum(wc, 0)])
+        # suffix sums within each ray (float64 segmented cumsum; the scan runs along the
+        # contiguous dimension of a [3, K] tensor)
+        wc = (w[None, :] * c.T).double()
+        cs = torch.cat([torch.zeros(3, 1, dtype=torch.float64, device=dev),
+                        torch.cumsum(wc, 1)], 1).T
         incl = cs[1:] - cs[row_off][ray]
         tot = (cs[row_off + cnt64] - cs[row_off])[ray]
         bg = rgba[:, 3].double()[ray] if ctx.white_back else torch.zeros_like(Tb, dtype=torch.float64)
@@ -113,8 +118,16 @@ class _FusedRender(Function):
         dbs = ds.sum().reshape(1)
         gp = torch.cat([dws, dbs, dWc.reshape(-1), dWv.reshape(-1), dbc, dW2.reshape(-1), db2,
                         dWo.reshape(-1), dbo])
-        gv = calls.integrate_backward(ctx.vshape, pin, pout, dF.contiguous())
-        return gv, gp, None, None, None, None, None, None, None
+        if ctx.accumulate is not None:
+            # scatter straight into the parameter's gradient: no dense temporary of the grid
+            leaf = ctx.accumulate
+            if leaf.grad is None:
+                leaf.grad = torch.zeros_like(leaf)
+            calls.integrate_backward(ctx.vshape, pin, pout, dF.contiguous(), leaf.grad)
+            gv = None
+        else:
+            gv = calls.integrate_backward(ctx.vshape, pin, pout, dF.contiguous())
+        return gv, gp, None, None, None, None, None, None, None, None
 
 
 class BenchmarkDecoder(nn.Module):
@@ -150,7 +163,8 @@ class FusedRenderer(nn.Module):
     adjoint.  ``voxels`` [(N+1)^3, C] and the decoder are parameters; the occupancy mask (and its
     acceleration structures) is fixed."""
 
-    def __init__(self, grid, decoder_module: BenchmarkDecoder, traversal: str = 'offline'):
+    def __init__(self, grid, decoder_module: BenchmarkDecoder, traversal: str = 'offline',
+                 accumulate_grad: bool = True):
         super().__init__()
         self.voxels = nn.Parameter(grid.voxels)
         grid.voxels = self.voxels.data  # the kernels read the parameter's storage
@@ -160,9 +174,10 @@ class FusedRenderer(nn.Module):
                                  decoder_module.canonical().detach().cpu().numpy())
         self.handle = Decoder(canon)
         self.traversal = traversal
+        self.accumulate_grad = accumulate_grad
 
     def forward(self, o, d, t_stop: float = 1e-4, white_back: bool = True):
         self.grid.voxels = self.voxels.data
         return _FusedRender.apply(self.voxels, self.decoder.canonical(), self.grid, self.handle,
                                   o.contiguous(), d.contiguous(), t_stop, white_back,
-                                  self.traversal)
+                                  self.traversal, self.accumulate_grad)

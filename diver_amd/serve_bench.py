@@ -1,12 +1,15 @@
 """Serving benchmark: Poisson frame arrivals at mixed resolutions over three resident scenes.
 
     python -m diver_amd.serve_bench [--rate 400] [--seconds 4] [--precision bf16]
+    python -m diver_amd.serve_bench --baseline      # one request at a time, FP32 decoder
 
 Scenes (synthetic, seeded): 128^3 x 32 channels, 256^3 x 32, 256^3 x 64, each with the benchmark
-decoder of its width.  Resolutions 200^2, 400^2, 800^2 and 1920x1080 are drawn with
-probabilities 0.4 / 0.3 / 0.2 / 0.1, scenes uniformly, cameras from the hemisphere set.  Prints
-one JSON line: throughput (rays/s over the run), p50/p99 frame latency overall and per
-resolution, the unloaded latency of each resolution alone, and launch counts.
+decoder of its width.  Resolutions 200^2, 400^2, 800^2 and 1920x1080 and the scenes are drawn
+uniformly, cameras from the hemisphere set, arrival times from a Poisson process (seed 4).  The
+served configuration packs up to 16 pending requests of a scene into one launch with two launches
+in flight; ``--baseline`` serves the same arrivals one request per launch, one launch at a time,
+with the FP32 decoder.  Prints one JSON line: throughput (rays/s over the run), p50/p99 frame
+latency overall and per resolution, the unloaded latency of each resolution alone, launch counts.
 """
 from __future__ import annotations
 
@@ -23,7 +26,7 @@ from .decoder import Decoder, from_baseline
 from .renderer import SceneGrid
 from .server import RenderServer
 
-RESOLUTIONS = [((200, 200), 0.4), ((400, 400), 0.3), ((800, 800), 0.2), ((1920, 1080), 0.1)]
+RESOLUTIONS = [((200, 200), 0.25), ((400, 400), 0.25), ((800, 800), 0.25), ((1920, 1080), 0.25)]
 SCENES = [('s128c32', 128, 32), ('s256c32', 256, 32), ('s256c64', 256, 64)]
 
 
@@ -42,13 +45,18 @@ def main(argv=None):
     ap.add_argument('--seconds', type=float, default=4.0)
     ap.add_argument('--precision', default='bf16')
     ap.add_argument('--streams', type=int, default=2)
+    ap.add_argument('--baseline', action='store_true',
+                    help='one request per launch, one launch in flight, FP32 decoder')
     args = ap.parse_args(argv)
+    max_requests = 16
+    if args.baseline:
+        args.precision, args.streams, max_requests = 'fp32', 1, 1
     dev = torch.device('cuda', 0)
     print('[serve] building scenes', file=sys.stderr, flush=True)
     scenes = build_scenes(dev)
     cams = S.hemisphere_cameras()
     rng = np.random.default_rng(S.SEED_ARRIVALS)
-    with RenderServer(dev, n_streams=args.streams) as srv:
+    with RenderServer(dev, n_streams=args.streams, max_requests=max_requests) as srv:
         for name, (g, d) in scenes.items():
             srv.add_scene(name, g, d, precision=args.precision)
         # unloaded latency of each resolution (scene 256^3 x 32, one frame at a time)
@@ -80,7 +88,8 @@ def main(argv=None):
             r.result(timeout=120)
         st = srv.stats(since=t_start)
     st.update({'offered_rate_hz': args.rate, 'seconds': args.seconds, 'precision': args.precision,
-               'streams': args.streams, 'unloaded_p50_ms': alone,
+               'streams': args.streams, 'max_requests_per_launch': max_requests,
+               'unloaded_p50_ms': alone,
                'scenes': [f'{n}^3 x {c}' for _name, n, c in SCENES]})
     print(json.dumps(st))
 

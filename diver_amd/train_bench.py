@@ -1,0 +1,152 @@
+"""Training-step benchmark: the fused renderer's hand-written adjoint (recompute) against the
+reference's autograd structure (stored samples), on one GPU.
+
+    python -m diver_amd.train_bench [--steps 20] [--warmup 3] [--rays 8192]
+
+Scene: the 256^3 x 32-channel sphere/shell grid with trainable corner features and a benchmark
+decoder of hidden width 64 (density softplus(F[0]); colour MLP (31 + 27) -> 64 -> 64 -> 3).
+Each step draws 8192 random rays from the 200 hemisphere cameras (800x800), renders them, takes
+the MSE against colours rendered from a second seeded grid, and runs Adam on features and
+decoder.  Both paths composite every interval (no early termination, as the reference's offline
+training forward, model/diver.py:183-207).
+
+  recompute: FusedRenderer -- traversal, integral, decoder and compositing in one kernel; the
+             backward re-walks each ray (diver_render_rows) instead of keeping samples.
+  stored:    masked_intersect -> integrate -> decoder -> composite with torch autograd over the
+             drop-in ops, keeping every interval's coordinates, features and activations, as the
+             reference's model/diver.py:141-207 does.
+
+Prints one JSON line: steps/s and the forward + backward device-memory high-water mark above the
+resident state (parameters, gradients, optimizer moments) for each path.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+import torch
+
+from . import calls
+from . import synthetic as S
+from .adjoint import BenchmarkDecoder, FusedRenderer, positional_encoding
+from .decoder import Decoder, from_baseline
+from .model.diver import composite
+from .renderer import SceneGrid
+from .utils.integrator import integrate
+from .utils.ray_voxel_intersection import masked_intersect
+
+
+def stored_forward(voxels, dec, mask, n, o, d):
+    """Reference-structured forward with autograd over stored samples."""
+    xyzmin, vs = -1.0, 2.0 / n
+    coord, hit, _tns = masked_intersect(o, d, xyzmin, 1.0, float(n), vs, mask, 1.0)
+    rows = coord[hit]
+    F = integrate(voxels, rows[:, :3].contiguous(), rows[:, 3:].contiguous())
+    ray = torch.where(hit)[0]
+    x = torch.cat([F[:, 1:], positional_encoding(d[ray])], -1)
+    x = torch.relu(torch.nn.functional.linear(x, dec.w1, dec.b1))
+    x = torch.relu(torch.nn.functional.linear(x, dec.w2, dec.b2))
+    col = torch.sigmoid(torch.nn.functional.linear(x, dec.w3, dec.b3))
+    B, M = hit.shape
+    sigma = torch.zeros(B, M, device=o.device).masked_scatter(hit, torch.nn.functional.softplus(F[:, 0]))
+    color = torch.zeros(B, M, 3, device=o.device).masked_scatter(hit[..., None].expand(B, M, 3),
+                                                                 col)
+    # composite() applies relu to sigma (model/diver.py:195); softplus output is already >= 0
+    rgb, _w = composite(sigma, color, hit, True)
+    return rgb
+
+
+def run(path, steps, warmup, n_rays, dev):
+    sc = S.config2_scene(32, 256)
+    gt = S.config2_scene(32, 256, seed=S.SEED_GT)
+    cams = S.hemisphere_cameras()
+    layers = S.baseline_decoder_layers(32, 64)
+    grid = SceneGrid.from_synthetic(sc, dev)
+    # targets: the ground-truth grid rendered by the same decoder with a shifted output bias
+    gt_grid = SceneGrid.from_synthetic(gt, dev)
+    gt_layers = [(w, b.copy()) for w, b in layers]
+    gt_layers[-1] = (gt_layers[-1][0], gt_layers[-1][1] + 0.5)
+    gt_dec = Decoder(from_baseline(gt_layers, 32))
+    batches = []
+    for i in range(warmup + steps):
+        o, d = S.random_rays(cams, n_rays, 800, 800, 100 + i)
+        og, dg = torch.from_numpy(o).to(dev), torch.from_numpy(d).to(dev)
+        opts = calls.render_opts('fp32', 'offline', 0.0, 0.0)
+        rgba, _ = calls.render_rays(gt_grid, gt_dec, og, dg, opts, with_count=False)
+        batches.append((og, dg, (rgba[:, :3] + rgba[:, 3:]).clone()))
+    del gt_grid, gt_dec
+    dec = BenchmarkDecoder(layers, 32).to(dev)
+    if path == 'recompute':
+        model = FusedRenderer(grid, dec, traversal='offline')
+        params = list(model.parameters())
+        fwd = lambda o, d: model(o, d, 0.0)
+    else:
+        voxels = torch.nn.Parameter(grid.voxels.clone())
+        del grid
+        mask = torch.from_numpy(sc.mask).to(dev)
+        params = [voxels] + list(dec.parameters())
+        fwd = lambda o, d: stored_forward(voxels, dec, mask, 256, o, d)
+    opt = torch.optim.Adam(params, lr=1e-3)
+
+    def step(b, track=None):
+        og, dg, tg = b
+        opt.zero_grad(set_to_none=False)
+        if track is not None:
+            torch.cuda.synchronize()
+            torch.cuda.reset_peak_memory_stats(dev)
+            before = torch.cuda.memory_allocated(dev)
+        loss = torch.mean((fwd(og, dg) - tg) ** 2)
+        loss.backward()
+        if track is not None:
+            torch.cuda.synchronize()
+            track.append(torch.cuda.max_memory_allocated(dev) - before)
+        opt.step()
+        return loss
+
+    for i in range(warmup):
+        step(batches[i])
+    torch.cuda.synchronize()
+    base = torch.cuda.memory_allocated(dev)
+    t0 = time.perf_counter()
+    losses = []
+    for i in range(steps):
+        losses.append(step(batches[warmup + i]))
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    # forward + backward memory high-water mark above the resident state (parameters,
+    # gradients, optimizer moments), on one more batch outside the timed loop
+    track = []
+    step(batches[-1], track)
+    return {'steps_per_s': steps / dt, 'ms_per_step': 1e3 * dt / steps,
+            'rays_per_s': steps * n_rays / dt, 'resident_bytes': base,
+            'fwd_bwd_peak_above_resident_bytes': track[0],
+            'loss_first': float(losses[0]), 'loss_last': float(losses[-1])}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--steps', type=int, default=20)
+    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--rays', type=int, default=8192)
+    ap.add_argument('--path', choices=['both', 'recompute', 'stored'], default='both')
+    args = ap.parse_args(argv)
+    dev = torch.device('cuda', 0)
+    out = {'workload': 'training step, 256^3 x 32 trainable grid, hidden-64 benchmark decoder, '
+                       f'{args.rays} random rays, MSE + Adam, no early termination',
+           'steps': args.steps}
+    paths = ('recompute', 'stored') if args.path == 'both' else (args.path,)
+    for path in paths:
+        print(f'[train] {path}', file=sys.stderr, flush=True)
+        out[path] = run(path, args.steps, args.warmup, args.rays, dev)
+        torch.cuda.empty_cache()
+    if len(paths) == 2:
+        out['speedup_recompute_vs_stored'] = (out['recompute']['steps_per_s'] /
+                                              out['stored']['steps_per_s'])
+    print(json.dumps(out))
+
+
+if __name__ == '__main__':
+    main()
