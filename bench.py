@@ -19,6 +19,17 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the render kernel on this workload, from the committed rocprofv3
+# FETCH_SIZE / WRITE_SIZE passes (scaled by a 2 GiB copy-kernel calibration)
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles',
+                            'render_800x800_256cube_traffic.json')
+
+
+def measured_traffic(n, width, height, channels):
+    if (n, width, height, channels) != (256, 800, 800, 32) or not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        return json.load(f)['bf16']['traffic_bytes_per_launch']
 
 
 def log(msg):
@@ -207,7 +218,7 @@ def main():
         'intervals_per_frame': i_eval,
         'roofline': {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS,
                      'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4),
-                     'traffic': None},
+                     'traffic': measured_traffic(args.n, W, H, C)},
         'cpu_baseline': cpu,
     }
     line.update(result_extra)
