@@ -269,14 +269,28 @@ __device__ __forceinline__ void camera_ray(const CameraDesc& cam, int x, int y, 
     oz = m[11];
 }
 
-// Ray r of a camera render in tile order: 8x8 pixel tiles, row-major over the image.
+// Ray r of a camera render in tile order: 8x8 pixel tiles, left to right within a row of tiles,
+// rows of tiles from the image centre outwards (centre, centre + 1, centre - 1, centre + 2, ...).
+// Workgroups start in id order, so the tiles whose rays usually cross the most of the scene start
+// first; the order changes no pixel's value.
 __device__ __forceinline__ bool camera_pixel(const CameraDesc& cam, int64_t r, int& x, int& y) {
-    int tiles_x = (cam.width + 7) >> 3;
-    int64_t tile = r >> 6;
-    int in = (int)(r & 63);
+    const int tiles_x = (cam.width + 7) >> 3;
+    const int tiles_y = (cam.height + 7) >> 3;
+    const int64_t tile = r >> 6;
+    const int in = (int)(r & 63);
+    const int k = (int)(tile / tiles_x);  // k-th row of tiles in centre-out order
+    const int c = (tiles_y - 1) >> 1, below = c, above = tiles_y - 1 - c;
+    const int m = below < above ? below : above;
+    int ty;
+    if (k == 0)
+        ty = c;
+    else if (k <= 2 * m)
+        ty = (k & 1) ? c + ((k + 1) >> 1) : c - (k >> 1);
+    else
+        ty = above > below ? c + m + (k - 2 * m) : c - m - (k - 2 * m);
     x = (int)(tile % tiles_x) * 8 + (in & 7);
-    y = (int)(tile / tiles_x) * 8 + (in >> 3);
-    return x < cam.width && y < cam.height;
+    y = ty * 8 + (in >> 3);
+    return k < tiles_y && x < cam.width && y < cam.height;
 }
 
 struct Kahan {
