@@ -230,22 +230,31 @@ __device__ __forceinline__ void segment_corners(float px, float py, float pz, fl
     }
 }
 
+// f[i] = sum over corners j = 0..7 (in that order) of w[j] * voxels[off[j] + i].  The channels go
+// in chunks of 4: the 8 loads of a chunk (8 corners x 16 B) are issued before the first is used,
+// and the chunks are kept apart so the loads in flight fit the register budget without spills.
 template <int C, typename I>
 __device__ __forceinline__ void gather_integral(const float* __restrict__ voxels, const float (&w)[8],
                                                 const I (&off)[8], float (&f)[C]) {
+    static_assert(C % 4 == 0, "channel count must be a multiple of 4");
 #pragma unroll
-    for (int i = 0; i < C; ++i) f[i] = 0.0f;
+    for (int c = 0; c < C / 4; ++c) {
+        float4 q[8];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        const float4* src = reinterpret_cast<const float4*>(voxels + off[j]);
+        for (int j = 0; j < 8; ++j) q[j] = reinterpret_cast<const float4*>(voxels + off[j])[c];
+        float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
 #pragma unroll
-        for (int i = 0; i < C / 4; ++i) {
-            float4 q = src[i];
-            f[4 * i + 0] += w[j] * q.x;
-            f[4 * i + 1] += w[j] * q.y;
-            f[4 * i + 2] += w[j] * q.z;
-            f[4 * i + 3] += w[j] * q.w;
+        for (int j = 0; j < 8; ++j) {
+            s0 += w[j] * q[j].x;
+            s1 += w[j] * q[j].y;
+            s2 += w[j] * q[j].z;
+            s3 += w[j] * q[j].w;
         }
+        f[4 * c + 0] = s0;
+        f[4 * c + 1] = s1;
+        f[4 * c + 2] = s2;
+        f[4 * c + 3] = s3;
+        __builtin_amdgcn_sched_barrier(0);
     }
 }
 

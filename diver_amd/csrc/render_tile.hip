@@ -34,6 +34,11 @@ constexpr int kQueue = 96;       // rows per wave queue (phase A stops at >= 32,
 constexpr int kFill = 32;        // phase A hands over to phase B at this many queued rows
 constexpr int kRowFloats = 8;    // p_in (3), p_out (3), weight, owner (as int bits)
 constexpr int kMaxLdsBlockWords = 8192;
+// One wave per workgroup: a workgroup's LDS is released only when its last wave ends, so with
+// several tiles per group a long tile would hold the LDS of its finished neighbours; single-wave
+// groups also let the dispatcher spread consecutive tiles over the XCDs one by one.
+constexpr int kWavesPerGroup = 1;
+constexpr int kBlocksInLds = 1;    // 8^3-block bitmap copied to LDS (1) or read through L1 (0)
 
 __host__ __device__ constexpr int wave_lds_floats(int H) { return kQueue * kRowFloats + 64 * 3 + 64 * H; }
 
@@ -99,18 +104,20 @@ __device__ __forceinline__ void shade_tile(const RenderArgs& a, const float* que
 }
 
 template <class D, int PREC, bool RT, bool DCH0>
-__global__ void __launch_bounds__(256, 3) render_tile_kernel(BatchArgs p) {
+__global__ void __launch_bounds__(64 * kWavesPerGroup, 3) render_tile_kernel(BatchArgs p) {
     static_assert(D::NPRE == 0, "the tile kernel needs density from the integrated feature");
     constexpr int C = D::C, H = D::H, Z = D::Z;
     const RenderArgs& a = p.g;
     extern __shared__ uint32_t smem[];
     const int n = a.n;
     const int nb = n >> 3;
-    const int nwords = (nb * nb * nb + 31) >> 5;
-    uint32_t* blocks = smem;
+    const int nwords = kBlocksInLds ? (nb * nb * nb + 31) >> 5 : 0;
+    const uint32_t* blocks = kBlocksInLds ? smem : a.blocks;
     float* wave_mem = reinterpret_cast<float*>(smem + ((nwords + 3) & ~3));
-    for (int i = threadIdx.x; i < nwords; i += blockDim.x) blocks[i] = a.blocks[i];
-    __syncthreads();
+    if constexpr (kBlocksInLds) {
+        for (int i = threadIdx.x; i < nwords; i += blockDim.x) smem[i] = a.blocks[i];
+        __syncthreads();
+    }
 
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
@@ -283,19 +290,20 @@ __global__ void __launch_bounds__(256, 3) render_tile_kernel(BatchArgs p) {
 
 static size_t render_tile_lds_bytes(int n, int H) {
     int nb = n >> 3;
-    int64_t nwords = ((int64_t)nb * nb * nb + 31) >> 5;
-    return (size_t)((nwords + 3) & ~3) * 4 + 4 * (size_t)wave_lds_floats(H) * sizeof(float);
+    int64_t nwords = kBlocksInLds ? ((int64_t)nb * nb * nb + 31) >> 5 : 0;
+    return (size_t)((nwords + 3) & ~3) * 4 + kWavesPerGroup * (size_t)wave_lds_floats(H) * sizeof(float);
 }
 
 bool tile_fits(int n) {
     const int nb = n >> 3;
     const int64_t nwords = ((int64_t)nb * nb * nb + 31) >> 5;
-    return n % 8 == 0 && nwords <= kMaxLdsBlockWords;
+    return n % 8 == 0 && (!kBlocksInLds || nwords <= kMaxLdsBlockWords);
 }
 
 template <class D, int PREC, bool RT>
 static hipError_t launch_t3(const BatchArgs& p, bool dch0, hipStream_t s) {
-    dim3 grid((unsigned)((p.total + 255) / 256)), block(256);
+    constexpr int threads = 64 * kWavesPerGroup;
+    dim3 grid((unsigned)((p.total + threads - 1) / threads)), block(threads);
     size_t lds = render_tile_lds_bytes(p.g.n, D::H);
     if (dch0)
         hipLaunchKernelGGL((render_tile_kernel<D, PREC, RT, true>), grid, block, lds, s, p);
