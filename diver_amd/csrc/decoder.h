@@ -154,6 +154,28 @@ __device__ __forceinline__ float density_act(float s) {
 
 // Density and opacity of one interval from the pre-activation s: sigma = act(s) and
 // alpha = 1 - exp(-sigma).  For softplus, exp(-softplus(s)) = 1 / (1 + e^s), so with
+// log(1 + e) for e in [0, 1], as 2 atanh(z) with z = e / (2 + e) in [0, 1/3]: z is corrected
+// for the rounding of 2 + e and of the reciprocal, then the odd series runs to z^13 (truncation
+// below 2^-26 relative).  Within 1.25 ulp of log1p over the whole interval, in 16 operations.
+__device__ __forceinline__ float log1p_unit(float e) {
+    const float d = 2.0f + e;
+    const float derr = e - (d - 2.0f);  // 2 + e = d + derr exactly
+    const float r = __builtin_amdgcn_rcpf(d);
+    float z = e * r;
+    float res = fmaf(-z, d, e);
+    res = fmaf(-z, derr, res);
+    z = fmaf(res, r, z);
+    const float z2 = z * z;
+    float p = 1.0f / 13.0f;
+    p = fmaf(p, z2, 1.0f / 11.0f);
+    p = fmaf(p, z2, 1.0f / 9.0f);
+    p = fmaf(p, z2, 1.0f / 7.0f);
+    p = fmaf(p, z2, 1.0f / 5.0f);
+    p = fmaf(p, z2, 1.0f / 3.0f);
+    const float t = 2.0f * z;
+    return fmaf(t * z2, p, t);
+}
+
 // e = exp(-|s|): alpha = 1 / (1 + e) for s > 0 and e / (1 + e) otherwise, which shares the
 // exponential with softplus.
 template <class D>
@@ -163,7 +185,7 @@ __device__ __forceinline__ void density_alpha(float s, float& sigma, float& alph
         alpha = -expm1f(-sigma);
     } else {
         const float e = __expf(-fabsf(s));
-        sigma = fmaxf(s, 0.0f) + log1pf(e);
+        sigma = fmaxf(s, 0.0f) + log1p_unit(e);
         alpha = (s > 0.0f ? 1.0f : e) * __builtin_amdgcn_rcpf(1.0f + e);
     }
 }

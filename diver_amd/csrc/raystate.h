@@ -220,13 +220,27 @@ __device__ __forceinline__ void segment_corners(float px, float py, float pz, fl
     w[1] = x01 - w[5] - w4;
     w[3] = w4 - w[7];
     const int cx1 = min(cx + 1, n), cy1 = min(cy + 1, n), cz1 = min(cz + 1, n);
-    const I sz = (I)(n + 1) * (I)(n + 1) * (I)C, sy = (I)(n + 1) * (I)C;
+    if constexpr (sizeof(I) == 4) {
+        // Corner index of (cx, cy, cz) with full-rate 24-bit multiplies ((n + 1)^3 C < 2^32 puts
+        // every operand below 2^24), then the +x/+y/+z neighbours by adding their strides (0 where
+        // the index is clamped at n).
+        const uint32_t np1 = (uint32_t)(n + 1);
+        const uint32_t base = __umul24(__umul24((uint32_t)cz, np1) + (uint32_t)cy, np1) + (uint32_t)cx;
+        const uint32_t dx = (uint32_t)(cx1 - cx);
+        const uint32_t dy = cy1 > cy ? np1 : 0u;
+        const uint32_t dz = cz1 > cz ? np1 * np1 : 0u;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        int gx = (j & 1) ? cx1 : cx;
-        int gy = ((j >> 1) & 1) ? cy1 : cy;
-        int gz = (j >> 2) ? cz1 : cz;
-        off[j] = (I)gz * sz + (I)gy * sy + (I)gx * (I)C;
+        for (int j = 0; j < 8; ++j)
+            off[j] = (I)((base + ((j & 1) ? dx : 0u) + ((j & 2) ? dy : 0u) + ((j & 4) ? dz : 0u)) * (uint32_t)C);
+    } else {
+        const I sz = (I)(n + 1) * (I)(n + 1) * (I)C, sy = (I)(n + 1) * (I)C;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            int gx = (j & 1) ? cx1 : cx;
+            int gy = ((j >> 1) & 1) ? cy1 : cy;
+            int gz = (j >> 2) ? cz1 : cz;
+            off[j] = (I)gz * sz + (I)gy * sy + (I)gx * (I)C;
+        }
     }
 }
 

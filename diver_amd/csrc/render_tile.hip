@@ -114,10 +114,6 @@ __global__ void __launch_bounds__(64 * kWavesPerGroup, 3) render_tile_kernel(Bat
     const int nwords = kBlocksInLds ? (nb * nb * nb + 31) >> 5 : 0;
     const uint32_t* blocks = kBlocksInLds ? smem : a.blocks;
     float* wave_mem = reinterpret_cast<float*>(smem + ((nwords + 3) & ~3));
-    if constexpr (kBlocksInLds) {
-        for (int i = threadIdx.x; i < nwords; i += blockDim.x) smem[i] = a.blocks[i];
-        __syncthreads();
-    }
 
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
@@ -138,18 +134,32 @@ __global__ void __launch_bounds__(64 * kWavesPerGroup, 3) render_tile_kernel(Bat
         dx = dy = 0.f;
         dz = 1.f;
     }
-    {
+    RayWalk ray;
+    if constexpr (RT)
+        ray.init_rt(ox, oy, oz, dx, dy, dz, a.xyzmin, a.voxel_size, n, a.octrees, valid);
+    else
+        ray.init(ox, oy, oz, dx, dy, dz, a.xyzmin, a.voxel_size, (float)n, valid);
+
+    // Most tiles of a view miss the occupied cells entirely: such a wave skips the bitmap copy and
+    // the view term (nothing would read them) and goes straight to writing T = 1.
+    const bool live = kWavesPerGroup > 1 || wave_any(ray.alive);
+    if (kBlocksInLds && live) {
+        if ((nwords & 3) == 0 && (reinterpret_cast<uintptr_t>(a.blocks) & 15) == 0) {
+            const uint4* src = reinterpret_cast<const uint4*>(a.blocks);
+            uint4* dst = reinterpret_cast<uint4*>(smem);
+            for (int i = threadIdx.x; i < (nwords >> 2); i += blockDim.x) dst[i] = src[i];
+        } else {
+            for (int i = threadIdx.x; i < nwords; i += blockDim.x) smem[i] = a.blocks[i];
+        }
+        __syncthreads();
+    }
+    if (live) {
         float hv[H];
         head_view<D>(a.W, dx, dy, dz, hv);
 #pragma unroll
         for (int j = 0; j < H; j += 4)
             *reinterpret_cast<float4*>(hvs + lane * H + j) = make_float4(hv[j], hv[j + 1], hv[j + 2], hv[j + 3]);
     }
-    RayWalk ray;
-    if constexpr (RT)
-        ray.init_rt(ox, oy, oz, dx, dy, dz, a.xyzmin, a.voxel_size, n, a.octrees, valid);
-    else
-        ray.init(ox, oy, oz, dx, dy, dz, a.xyzmin, a.voxel_size, (float)n, valid);
     OccCache occ;
 
     Kahan S;
