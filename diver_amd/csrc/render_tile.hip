@@ -39,8 +39,63 @@ constexpr int kMaxLdsBlockWords = 8192;
 // groups also let the dispatcher spread consecutive tiles over the XCDs one by one.
 constexpr int kWavesPerGroup = 1;
 constexpr int kBlocksInLds = 1;    // 8^3-block bitmap copied to LDS (1) or read through L1 (0)
+constexpr int kGatherChunk = 4;    // channels per round of corner loads in the packed gather (4 or 8)
 
 __host__ __device__ constexpr int wave_lds_floats(int H) { return kQueue * kRowFloats + 64 * 3 + 64 * H; }
+
+// Segment integral of one row, packed for the matrix cores: zp[i] holds channels 2i and 2i+1
+// rounded to PREC.  Each channel is the FP32 sum over corners j = 0..7 (in that order) of
+// w[j] * voxels[off[j] + channel], exactly as gather_integral computes it; packing each chunk as
+// soon as its sums are done halves the registers the row occupies, which leaves room for the
+// loads of a chunk (8 corners x CH channels) to be in flight together.
+template <int C, int PREC, int CH>
+__device__ __forceinline__ void gather_packed(const float* __restrict__ voxels, const float (&w)[8],
+                                              const uint32_t (&off)[8], uint32_t (&zp)[C / 2]) {
+    static_assert(CH == 4 || CH == 8, "chunk of 4 or 8 channels");
+    static_assert(C % CH == 0, "channel count must be a multiple of the chunk");
+#pragma unroll
+    for (int c = 0; c < C / CH; ++c) {
+        float4 q[8][CH / 4];
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+#pragma unroll
+            for (int h = 0; h < CH / 4; ++h)
+                q[j][h] = reinterpret_cast<const float4*>(voxels + off[j])[c * (CH / 4) + h];
+#pragma unroll
+        for (int h = 0; h < CH / 4; ++h) {
+            float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                s0 += w[j] * q[j][h].x;
+                s1 += w[j] * q[j][h].y;
+                s2 += w[j] * q[j][h].z;
+                s3 += w[j] * q[j][h].w;
+            }
+            zp[(c * CH + 4 * h) / 2] = pack2<PREC>(s0, s1);
+            zp[(c * CH + 4 * h) / 2 + 1] = pack2<PREC>(s2, s3);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+// The packed row as the two B operands per 16-channel k chunk (rows_to_b's layout).
+template <int K>
+__device__ __forceinline__ void packed_to_b(const uint32_t (&zp)[K / 2], u32x4_t (&b0)[K / 16],
+                                            u32x4_t (&b1)[K / 16]) {
+#pragma unroll
+    for (int kc = 0; kc < K / 16; ++kc) {
+        uint32_t x[4], y[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            x[q] = zp[8 * kc + q];
+            y[q] = zp[8 * kc + 4 + q];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) swap32(x[q], y[q]);
+        b0[kc] = u32x4_t{x[0], x[1], x[2], x[3]};
+        b1[kc] = u32x4_t{y[0], y[1], y[2], y[3]};
+    }
+}
 
 template <class D, int PREC>
 __device__ __forceinline__ void shade_tile(const RenderArgs& a, const float* queue,
@@ -237,20 +292,17 @@ __global__ void __launch_bounds__(64 * kWavesPerGroup, 3) render_tile_kernel(Bat
         __builtin_amdgcn_wave_barrier();
         const int nbatch = qn < 64 ? qn : 64;
         const bool mine = lane < nbatch;
-        float z[Z];
+        float w[8];
+        uint32_t off[8];
         if (mine) {
             float4 r0 = *reinterpret_cast<const float4*>(queue + lane * kRowFloats);
             float4 r1 = *reinterpret_cast<const float4*>(queue + lane * kRowFloats + 4);
-            float w[8];
-            uint32_t off[8];
             segment_corners<C, uint32_t>(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, n, w, off);
-            gather_integral<C>(a.voxels, w, off, z);
-        } else {
-#pragma unroll
-            for (int i = 0; i < Z; ++i) z[i] = 0.0f;
         }
         if constexpr (PREC == kPrecFP32) {
+            float z[Z];
             if (mine) {
+                gather_integral<C>(a.voxels, w, off, z);
                 const int owner = __builtin_bit_cast(int, queue[lane * kRowFloats + 7]);
                 float hvr[H];
 #pragma unroll
@@ -263,8 +315,16 @@ __global__ void __launch_bounds__(64 * kWavesPerGroup, 3) render_tile_kernel(Bat
                 atomicAdd(&acc[owner * 3 + 2], wgt * rgb[2]);
             }
         } else {
+            static_assert(Z == C, "the packed gather feeds the head input directly");
+            uint32_t zp[Z / 2];
+            if (mine) {
+                gather_packed<C, PREC, kGatherChunk>(a.voxels, w, off, zp);
+            } else {
+#pragma unroll
+                for (int i = 0; i < Z / 2; ++i) zp[i] = 0u;  // +0 in both halves
+            }
             u32x4_t b0[Z / 16], b1[Z / 16];
-            rows_to_b<PREC, Z>(z, b0, b1);
+            packed_to_b<Z>(zp, b0, b1);
             shade_tile<D, PREC>(a, queue, hvs, acc, 0, nbatch, b0, lane);
             if (nbatch > 32) shade_tile<D, PREC>(a, queue, hvs, acc, 1, nbatch, b1, lane);
         }
