@@ -156,6 +156,8 @@ struct RayWalk {
 struct OccCache {
     int brick = -1;
     uint32_t lo = 0, hi = 0;
+    int block = -1;            // 8^3 block of the cached brick
+    bool block_empty = false;  // that block has no occupied voxel
 
     __device__ __forceinline__ bool lookup(const uint8_t* __restrict__ mask,
                                            const uint32_t* blocks,
@@ -168,7 +170,9 @@ struct OccCache {
                 brick = br;
                 const int nb = n >> 3;
                 int bid = ((vz >> 3) * nb + (vy >> 3)) * nb + (vx >> 3);
-                if ((blocks[bid >> 5] >> (bid & 31)) & 1u) {
+                block = bid;
+                block_empty = !((blocks[bid >> 5] >> (bid & 31)) & 1u);
+                if (!block_empty) {
                     unsigned long long w = bricks[br];
                     lo = (uint32_t)w;
                     hi = (uint32_t)(w >> 32);

@@ -40,6 +40,7 @@ constexpr int kMaxLdsBlockWords = 8192;
 constexpr int kWavesPerGroup = 1;
 constexpr int kBlocksInLds = 1;    // 8^3-block bitmap copied to LDS (1) or read through L1 (0)
 constexpr int kGatherChunk = 4;    // channels per round of corner loads in the packed gather (4 or 8)
+constexpr int kSkip = 2;           // cheap steps per walk iteration over voxels known to be empty (0: off)
 
 __host__ __device__ constexpr int wave_lds_floats(int H) { return kQueue * kRowFloats + 64 * 3 + 64 * H; }
 
@@ -158,6 +159,49 @@ __device__ __forceinline__ void shade_tile(const RenderArgs& a, const float* que
     }
 }
 
+// Cheap steps over voxels already known to be empty.  Before each full walk iteration a lane
+// takes exact DDA steps for as long as the new voxel lies inside the grid, in its cached 4^3
+// brick (or in the cached brick's 8^3 block when that block is empty) and has its occupancy bit
+// clear: the full iteration would only have stepped over such a voxel.  A step that fails the
+// test is not taken here; the full iteration takes it again (same operands, same result).  Up to
+// kSkip steps per call (more lets lanes that cannot skip wait longer); lanes stop one by one.
+template <bool RT>
+__device__ __forceinline__ void skip_empty(RayWalk& ray, const OccCache& occ, int n) {
+    const float nf = (float)n;
+    bool sk = ray.alive && occ.brick >= 0;
+#pragma unroll 1
+    for (int k = 0; k < kSkip; ++k) {
+        sk = sk && ray.steps < ray.max_steps;
+        if (!wave_any(sk)) break;
+        float nx, ny, nz;
+        dda_step_r(ray.lx, ray.ly, ray.lz, ray.dx, ray.dy, ray.dz, ray.rx, ray.ry, ray.rz, ray.sx,
+                   ray.sy, ray.sz, nx, ny, nz);
+        if (wave_any(sk && !ray.fast)) {
+            if (!ray.fast)
+                dda_step(ray.lx, ray.ly, ray.lz, ray.dx, ray.dy, ray.dz, ray.sx, ray.sy, ray.sz, nx,
+                         ny, nz);
+        }
+        int vx, vy, vz;
+        segment_voxel1(ray.lx, ray.ly, ray.lz, nx, ny, nz, vx, vy, vz);
+        const bool inr = (unsigned)vx < (unsigned)n && (unsigned)vy < (unsigned)n &&
+                         (unsigned)vz < (unsigned)n;
+        const uint32_t n4 = (uint32_t)n >> 2, n8 = (uint32_t)n >> 3;
+        const uint32_t ux = (uint32_t)vx, uy = (uint32_t)vy, uz = (uint32_t)vz;
+        const uint32_t region = occ.block_empty
+                                    ? __umul24(__umul24(uz >> 3, n8) + (uy >> 3), n8) + (ux >> 3)
+                                    : __umul24(__umul24(uz >> 2, n4) + (uy >> 2), n4) + (ux >> 2);
+        const int cached = occ.block_empty ? occ.block : occ.brick;
+        const uint32_t bit = ((uz & 3u) << 4) | ((uy & 3u) << 2) | (ux & 3u);
+        const bool empty = occ.block_empty || !(((bit < 32u ? occ.lo : occ.hi) >> (bit & 31u)) & 1u);
+        sk = sk && inr && (int)region == cached && empty &&
+             !(RT ? outside_open(nx, ny, nz, nf) : outside(nx, ny, nz, nf));
+        ray.lx = sk ? nx : ray.lx;
+        ray.ly = sk ? ny : ray.ly;
+        ray.lz = sk ? nz : ray.lz;
+        ray.steps += sk ? 1 : 0;
+    }
+}
+
 template <class D, int PREC, bool RT, bool DCH0>
 __global__ void __launch_bounds__(64 * kWavesPerGroup, 3) render_tile_kernel(BatchArgs p) {
     static_assert(D::NPRE == 0, "the tile kernel needs density from the integrated feature");
@@ -227,6 +271,7 @@ __global__ void __launch_bounds__(64 * kWavesPerGroup, 3) render_tile_kernel(Bat
         // ---------------- phase A: walk, density, transmittance, enqueue ------------------------
         while (qn < kFill) {
             if (!wave_any(ray.alive)) break;
+            if constexpr (kSkip > 0) skip_empty<RT>(ray, occ, n);
             bool push = false;
             float px = 0.f, py = 0.f, pz = 0.f, qx = 0.f, qy = 0.f, qz = 0.f, wgt = 0.f;
             if (ray.alive) {
