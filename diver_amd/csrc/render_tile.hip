@@ -348,6 +348,11 @@ __global__ void __launch_bounds__(64 * kWavesPerGroup, 3) render_tile_kernel(Bat
             float z[Z];
             if (mine) {
                 gather_integral<C>(a.voxels, w, off, z);
+            } else {
+#pragma unroll
+                for (int i = 0; i < Z; ++i) z[i] = 0.0f;
+            }
+            if (mine) {
                 const int owner = __builtin_bit_cast(int, queue[lane * kRowFloats + 7]);
                 float hvr[H];
 #pragma unroll
