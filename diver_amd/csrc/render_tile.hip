@@ -202,8 +202,13 @@ __device__ __forceinline__ void skip_empty(RayWalk& ray, const OccCache& occ, in
     }
 }
 
+// Register budget: 3 waves per SIMD for hidden width 32.  Wider decoders keep 64 x H view terms
+// in LDS, which already limits a CU to 6 waves (1.5 per SIMD), so they take the 2-wave budget.
+template <class D>
+constexpr int tile_waves_per_simd() { return D::H > 32 ? 2 : 3; }
+
 template <class D, int PREC, bool RT, bool DCH0>
-__global__ void __launch_bounds__(64 * kWavesPerGroup, 3) render_tile_kernel(BatchArgs p) {
+__global__ void __launch_bounds__(64 * kWavesPerGroup, tile_waves_per_simd<D>()) render_tile_kernel(BatchArgs p) {
     static_assert(D::NPRE == 0, "the tile kernel needs density from the integrated feature");
     constexpr int C = D::C, H = D::H, Z = D::Z;
     const RenderArgs& a = p.g;
