@@ -19,7 +19,7 @@ EXPORTS = [
     'diver_composite', 'diver_composite_backward', 'diver_decoder_create', 'diver_decoder_destroy',
     'diver_decoder_param_count', 'diver_mlp_eval', 'diver_render_rays', 'diver_render_camera',
     'diver_camera_rays', 'diver_block_bits', 'diver_walk_rays', 'diver_render_rows',
-    'diver_decoder_update', 'diver_brick_bits', 'diver_render_batch',
+    'diver_decoder_update', 'diver_brick_bits', 'diver_render_batch', 'diver_alpha_map',
 ]
 
 MAX_REQUESTS = 16
@@ -92,6 +92,7 @@ def _declare(L):
                               vp, vp, vp, vp],
         'diver_render_batch': [ctypes.POINTER(Grid), vp, ctypes.POINTER(RenderOpts),
                                ctypes.POINTER(Request), i32, vp],
+        'diver_alpha_map': [ctypes.POINTER(Grid), vp, i64, vp, vp, i32, vp, vp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

@@ -264,6 +264,22 @@ def render_rays(grid, decoder, o, d, opts, with_count=True):
     return rgba, count
 
 
+def alpha_map(grid, decoder, o, d, pad=0, out=None):
+    """Max rendering weight per voxel over rays (o, d): float32 [N^3] at x + y N + z N^2, walked
+    over grid.mask.  Accumulates into `out` when given (it must start at zero)."""
+    dev = _dev(grid.voxels, grid.mask, o, d)
+    o, d = _f32(o, 'o'), _f32(d, 'd')
+    n = grid.mask.shape[0]
+    if out is None:
+        out = torch.zeros(n ** 3, dtype=torch.float32, device=dev)
+    elif out.dtype != torch.float32 or out.numel() != n ** 3 or not out.is_contiguous():
+        raise ValueError('out must be a contiguous float32 tensor of N^3 elements')
+    g = _grid_of(grid)
+    _lib.call('diver_alpha_map', ctypes.byref(g), decoder.handle, o.shape[0], o.data_ptr(),
+              d.data_ptr(), int(pad), out.data_ptr(), _s(o))
+    return out
+
+
 def render_camera(grid, decoder, c2w, width, height, focal, opts, with_count=True, out=None):
     dev = _dev(grid.voxels, grid.mask)
     rgba = out if out is not None else torch.empty((height, width, 4), dtype=torch.float32,

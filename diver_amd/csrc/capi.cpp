@@ -443,6 +443,23 @@ int diver_render_rays(const diver_grid* grid, const diver_decoder* dec,
     return dispatch_render(dec, a, opts->precision, S(stream), "render_rays");
 }
 
+int diver_alpha_map(const diver_grid* grid, const diver_decoder* dec, int64_t n_rays,
+                    const float* o, const float* d, int pad, float* alpha_map,
+                    diver_stream_t stream) {
+    diver::RenderArgs a{};
+    const diver_render_opts opts{DIVER_FP32, DIVER_WALK_OFFLINE, 0.0f, 0.0f};
+    int st = render_common(grid, dec, &opts, a);
+    if (st != DIVER_OK) return st;
+    REQUIRE(n_rays >= 0, "n_rays < 0");
+    REQUIRE(n_rays == 0 || (o && d && alpha_map), "null pointer");
+    REQUIRE(pad >= 0, "pad < 0");
+    a.n_rays = n_rays;
+    a.o = o;
+    a.d = d;
+    return hip_status(diver::launch_alpha_map(dec->variant, a, pad, alpha_map, S(stream)),
+                      "alpha_map");
+}
+
 int diver_render_camera(const diver_grid* grid, const diver_decoder* dec,
                         const diver_render_opts* opts, const float* c2w_host, int width,
                         int height, float focal, float* rgba, int32_t* count,

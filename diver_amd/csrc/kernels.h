@@ -85,6 +85,7 @@ hipError_t launch_mlp_eval(int variant, int64_t batch, const float* coord, const
                            uint8_t* finish, const float* W, float alpha_skip, float t_stop,
                            hipStream_t s);
 hipError_t launch_render(int variant, const RenderArgs& a, int prec, hipStream_t s);
+hipError_t launch_alpha_map(int variant, const RenderArgs& a, int pad, float* map, hipStream_t s);
 hipError_t launch_render_tiles(int variant, const BatchArgs& p, int prec, bool dch0,
                                hipStream_t s);
 bool tile_fits(int n);  // the grid's block bitmap fits the tile kernel's LDS

@@ -190,6 +190,18 @@ int diver_render_rays(const diver_grid* grid, const diver_decoder* dec,
                       const diver_render_opts* opts, int64_t n_rays, const float* o,
                       const float* d, float* rgba, int32_t* count, diver_stream_t stream);
 
+/* Occupancy extraction (replaces the alpha-map loop of prune.py:83-143: masked_intersect or
+ * ray_voxel_intersect, DIVeR.decode, DIVeR.render's weights and torch_scatter's max).  Every
+ * occupied voxel a ray crosses (the offline walk over grid->mask; pass an all-ones mask for the
+ * unmasked traversal) gets the rendering weight alpha_i * T_i of that interval (no early
+ * termination, FP32 decoder); the voxel min(int(p_in + 1e-4), int(p_out + 1e-4)) per axis, if
+ * it lies in [pad, N-1-pad]^3, keeps the maximum in alpha_map [N^3] at x + y N + z N^2.  The
+ * map is accumulated: zero it once, then call per batch of rays.  Threshold it to get the new
+ * voxel mask (prune.py:146). */
+int diver_alpha_map(const diver_grid* grid, const diver_decoder* dec, int64_t n_rays,
+                    const float* o, const float* d, int pad, float* alpha_map,
+                    diver_stream_t stream);
+
 /* Same for a pinhole camera (utils/dataset/ray_utils.py:5-56 conventions: pixel centres +0.5,
  * -z forward, normalised directions); c2w_host is row-major 3x4.  rgba/count are [height][width]. */
 int diver_render_camera(const diver_grid* grid, const diver_decoder* dec,
