@@ -20,6 +20,7 @@ EXPORTS = [
     'diver_decoder_param_count', 'diver_mlp_eval', 'diver_render_rays', 'diver_render_camera',
     'diver_camera_rays', 'diver_block_bits', 'diver_walk_rays', 'diver_render_rows',
     'diver_decoder_update', 'diver_brick_bits', 'diver_render_batch', 'diver_alpha_map',
+    'diver_corner_keys', 'diver_integrate_table', 'diver_integrate_table_backward',
 ]
 
 MAX_REQUESTS = 16
@@ -93,6 +94,9 @@ def _declare(L):
         'diver_render_batch': [ctypes.POINTER(Grid), vp, ctypes.POINTER(RenderOpts),
                                ctypes.POINTER(Request), i32, vp],
         'diver_alpha_map': [ctypes.POINTER(Grid), vp, i64, vp, vp, i32, vp, vp],
+        'diver_corner_keys': [i32, i64, vp, vp, vp, vp],
+        'diver_integrate_table': [vp, i32, i64, vp, vp, vp, vp, vp],
+        'diver_integrate_table_backward': [i32, i64, vp, vp, vp, vp, vp, vp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

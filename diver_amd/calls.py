@@ -264,6 +264,43 @@ def render_rays(grid, decoder, o, d, opts, with_count=True):
     return rgba, count
 
 
+def corner_keys(n1, coord_in, coord_out):
+    """int64 [K, 8]: keys z n1^2 + y n1 + x of the 8 corners of each row's integral."""
+    dev = _dev(coord_in, coord_out)
+    ci, co = _f32(coord_in, 'coord_in'), _f32(coord_out, 'coord_out')
+    K = ci.shape[0]
+    keys = torch.empty((K, 8), dtype=torch.int64, device=dev)
+    _lib.call('diver_corner_keys', int(n1), K, ci.data_ptr(), co.data_ptr(), keys.data_ptr(), _s(ci))
+    return keys
+
+
+def integrate_table(table, corner_row, coord_in, coord_out):
+    """[K, C]: the segment integral with corner j of row r taken from table[corner_row[r, j]]."""
+    dev = _dev(table, corner_row, coord_in, coord_out)
+    table = _f32(table, 'table')
+    ci, co = _f32(coord_in, 'coord_in'), _f32(coord_out, 'coord_out')
+    if corner_row.dtype != torch.int64:
+        raise TypeError('corner_row must be int64')
+    crow = corner_row.contiguous()
+    K, C = ci.shape[0], table.shape[-1]
+    out = torch.empty((K, C), dtype=torch.float32, device=dev)
+    _lib.call('diver_integrate_table', table.data_ptr(), int(C), K, crow.data_ptr(), ci.data_ptr(),
+              co.data_ptr(), out.data_ptr(), _s(ci))
+    return out
+
+
+def integrate_table_backward(n_rows_table, corner_row, coord_in, coord_out, grad_feature):
+    """[U, C]: d feature / d table ^T grad_feature."""
+    dev = _dev(corner_row, coord_in, coord_out, grad_feature)
+    ci, co = _f32(coord_in, 'coord_in'), _f32(coord_out, 'coord_out')
+    g = _f32(grad_feature, 'grad_feature')
+    K, C = g.shape
+    gt = torch.zeros((int(n_rows_table), C), dtype=torch.float32, device=dev)
+    _lib.call('diver_integrate_table_backward', int(C), K, corner_row.contiguous().data_ptr(),
+              ci.data_ptr(), co.data_ptr(), g.data_ptr(), gt.data_ptr(), _s(ci))
+    return gt
+
+
 def alpha_map(grid, decoder, o, d, pad=0, out=None):
     """Max rendering weight per voxel over rays (o, d): float32 [N^3] at x + y N + z N^2, walked
     over grid.mask.  Accumulates into `out` when given (it must start at zero)."""

@@ -161,6 +161,39 @@ int diver_integrate_backward(int n1, int channels, int64_t k, const float* coord
                       "integrate_backward");
 }
 
+int diver_corner_keys(int n1, int64_t k, const float* coord_in, const float* coord_out,
+                      int64_t* keys, diver_stream_t stream) {
+    REQUIRE(k >= 0 && n1 > 1, "bad sizes");
+    REQUIRE(k == 0 || (coord_in && coord_out && keys), "null pointer");
+    return hip_status(diver::launch_corner_keys(n1, k, coord_in, coord_out, keys, S(stream)),
+                      "corner_keys");
+}
+
+int diver_integrate_table(const float* table, int channels, int64_t k, const int64_t* corner_row,
+                          const float* coord_in, const float* coord_out, float* feature,
+                          diver_stream_t stream) {
+    REQUIRE(k >= 0, "bad sizes");
+    REQUIRE(channels > 0 && channels % 4 == 0, "channels must be a positive multiple of 4");
+    REQUIRE(k == 0 || (table && corner_row && coord_in && coord_out && feature), "null pointer");
+    return hip_status(diver::launch_integrate_table(table, channels, k, corner_row, coord_in,
+                                                    coord_out, feature, S(stream)),
+                      "integrate_table");
+}
+
+int diver_integrate_table_backward(int channels, int64_t k, const int64_t* corner_row,
+                                   const float* coord_in, const float* coord_out,
+                                   const float* grad_feature, float* grad_table,
+                                   diver_stream_t stream) {
+    REQUIRE(k >= 0, "bad sizes");
+    REQUIRE(channels > 0 && channels % 4 == 0, "channels must be a positive multiple of 4");
+    REQUIRE(k == 0 || (corner_row && coord_in && coord_out && grad_feature && grad_table),
+            "null pointer");
+    return hip_status(diver::launch_integrate_table_backward(channels, k, corner_row, coord_in,
+                                                             coord_out, grad_feature, grad_table,
+                                                             S(stream)),
+                      "integrate_table_backward");
+}
+
 int diver_composite(int64_t batch, int m, const float* sigma, const float* color,
                     const uint8_t* mask, int white_back, float* rgb, float* weight,
                     diver_stream_t stream) {

@@ -121,6 +121,120 @@ __global__ void __launch_bounds__(256) integrate_backward_kernel(int n1, int C, 
     }
 }
 
+// ---- implicit grid (integrator.py:3-60): features come from a table of distinct corners -----
+
+// The 8 corners of each row's integral as keys z n1^2 + y n1 + x (corner j = (j&1, (j>>1)&1,
+// j>>2), clamped as integrate_kernel does).  The implicit stage evaluates its MLP once per
+// distinct key and integrates from that table.
+__global__ void __launch_bounds__(256) corner_keys_kernel(int n1, int64_t K,
+                                                          const float* __restrict__ ci,
+                                                          const float* __restrict__ co,
+                                                          int64_t* __restrict__ keys) {
+    const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= K) return;
+    const CornerWeights cw = corner_weights(ci[3 * row], ci[3 * row + 1], ci[3 * row + 2],
+                                            co[3 * row], co[3 * row + 1], co[3 * row + 2]);
+    const int xs[2] = {min(max(cw.cx, 0), n1 - 1), min(max(cw.cx + 1, 0), n1 - 1)};
+    const int ys[2] = {min(max(cw.cy, 0), n1 - 1), min(max(cw.cy + 1, 0), n1 - 1)};
+    const int zs[2] = {min(max(cw.cz, 0), n1 - 1), min(max(cw.cz + 1, 0), n1 - 1)};
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+        keys[8 * row + j] = ((int64_t)zs[j >> 2] * n1 + ys[(j >> 1) & 1]) * n1 + xs[j & 1];
+}
+
+// integrate_kernel with corner j of row r read from table row crow[8 r + j] ([U][C]).
+template <int LANES>
+__global__ void __launch_bounds__(256) integrate_table_kernel(const float* __restrict__ table,
+                                                              int C, int64_t K,
+                                                              const int64_t* __restrict__ crow,
+                                                              const float* __restrict__ ci,
+                                                              const float* __restrict__ co,
+                                                              float* __restrict__ out) {
+    const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t row = gt / LANES;
+    const int sub = (int)(gt % LANES);
+    if (row >= K) return;
+    const CornerWeights cw = corner_weights(ci[3 * row], ci[3 * row + 1], ci[3 * row + 2],
+                                            co[3 * row], co[3 * row + 1], co[3 * row + 2]);
+    for (int c4 = sub; c4 < C / 4; c4 += LANES) {
+        float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const float4 f = *reinterpret_cast<const float4*>(table + crow[8 * row + j] * C + 4 * c4);
+            const float w = cw.w[j];
+            acc.x += w * f.x;
+            acc.y += w * f.y;
+            acc.z += w * f.z;
+            acc.w += w * f.w;
+        }
+        *reinterpret_cast<float4*>(out + row * C + 4 * c4) = acc;
+    }
+}
+
+template <int LANES>
+__global__ void __launch_bounds__(256) integrate_table_backward_kernel(
+    int C, int64_t K, const int64_t* __restrict__ crow, const float* __restrict__ ci,
+    const float* __restrict__ co, const float* __restrict__ gout, float* __restrict__ gtable) {
+    const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t row = gt / LANES;
+    const int sub = (int)(gt % LANES);
+    if (row >= K) return;
+    const CornerWeights cw = corner_weights(ci[3 * row], ci[3 * row + 1], ci[3 * row + 2],
+                                            co[3 * row], co[3 * row + 1], co[3 * row + 2]);
+    for (int c4 = sub; c4 < C / 4; c4 += LANES) {
+        const float4 g = *reinterpret_cast<const float4*>(gout + row * C + 4 * c4);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            float* dst = gtable + crow[8 * row + j] * C + 4 * c4;
+            const float w = cw.w[j];
+            unsafeAtomicAdd(dst + 0, w * g.x);
+            unsafeAtomicAdd(dst + 1, w * g.y);
+            unsafeAtomicAdd(dst + 2, w * g.z);
+            unsafeAtomicAdd(dst + 3, w * g.w);
+        }
+    }
+}
+
+hipError_t launch_corner_keys(int n1, int64_t K, const float* ci, const float* co, int64_t* keys,
+                              hipStream_t s) {
+    if (K == 0) return hipSuccess;
+    hipLaunchKernelGGL(corner_keys_kernel, dim3((unsigned)((K + 255) / 256)), dim3(256), 0, s, n1,
+                       K, ci, co, keys);
+    return hipGetLastError();
+}
+
+#define DIVER_TABLE_LAUNCH(KERNEL, ...)                                                        \
+    do {                                                                                       \
+        int lanes = C / 4;                                                                     \
+        if (lanes > 16) lanes = 16;                                                            \
+        const int64_t threads = K * lanes;                                                     \
+        const dim3 grid((unsigned)((threads + 255) / 256));                                    \
+        switch (lanes) {                                                                       \
+            case 1: hipLaunchKernelGGL(KERNEL<1>, grid, dim3(256), 0, s, __VA_ARGS__); break;  \
+            case 2: hipLaunchKernelGGL(KERNEL<2>, grid, dim3(256), 0, s, __VA_ARGS__); break;  \
+            case 4: hipLaunchKernelGGL(KERNEL<4>, grid, dim3(256), 0, s, __VA_ARGS__); break;  \
+            case 8: hipLaunchKernelGGL(KERNEL<8>, grid, dim3(256), 0, s, __VA_ARGS__); break;  \
+            default: hipLaunchKernelGGL(KERNEL<16>, grid, dim3(256), 0, s, __VA_ARGS__); break; \
+        }                                                                                      \
+    } while (0)
+
+hipError_t launch_integrate_table(const float* table, int C, int64_t K, const int64_t* crow,
+                                  const float* ci, const float* co, float* out, hipStream_t s) {
+    if (K == 0) return hipSuccess;
+    DIVER_TABLE_LAUNCH(integrate_table_kernel, table, C, K, crow, ci, co, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_integrate_table_backward(int C, int64_t K, const int64_t* crow, const float* ci,
+                                           const float* co, const float* gout, float* gtable,
+                                           hipStream_t s) {
+    if (K == 0) return hipSuccess;
+    DIVER_TABLE_LAUNCH(integrate_table_backward_kernel, C, K, crow, ci, co, gout, gtable);
+    return hipGetLastError();
+}
+
+#undef DIVER_TABLE_LAUNCH
+
 hipError_t launch_integrate(const float* voxels, int n1, int C, int64_t K, const float* ci,
                             const float* co, float* out, hipStream_t s) {
     if (K == 0) return hipSuccess;

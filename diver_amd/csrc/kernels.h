@@ -86,6 +86,13 @@ hipError_t launch_mlp_eval(int variant, int64_t batch, const float* coord, const
                            hipStream_t s);
 hipError_t launch_render(int variant, const RenderArgs& a, int prec, hipStream_t s);
 hipError_t launch_alpha_map(int variant, const RenderArgs& a, int pad, float* map, hipStream_t s);
+hipError_t launch_corner_keys(int n1, int64_t K, const float* ci, const float* co, int64_t* keys,
+                              hipStream_t s);
+hipError_t launch_integrate_table(const float* table, int C, int64_t K, const int64_t* crow,
+                                  const float* ci, const float* co, float* out, hipStream_t s);
+hipError_t launch_integrate_table_backward(int C, int64_t K, const int64_t* crow, const float* ci,
+                                           const float* co, const float* gout, float* gtable,
+                                           hipStream_t s);
 hipError_t launch_render_tiles(int variant, const BatchArgs& p, int prec, bool dch0,
                                hipStream_t s);
 bool tile_fits(int n);  // the grid's block bitmap fits the tile kernel's LDS

@@ -5,8 +5,9 @@ mlp_point, mlp_view, dir_encode), same parameter/buffer names (``voxels``, ``vox
 ``mlp1.*``, ``mlp2.*``), and the same method signatures and return shapes for
 ``extract_features``, ``decode``, ``forward`` and ``render``.  Traversal, the segment integral and
 compositing run as libdiver kernels; the decoder MLPs stay torch ``nn.Linear`` layers as in the
-reference.  Only explicit grids are supported (the implicit stage, ``hparams.implicit``, is
-outside this path).
+reference.  With ``hparams.implicit`` the corner features come from the implicit voxel MLP
+(``mlp_voxel``, model/diver.py:32-34) through ``integrate_mlp`` until ``init_voxels`` bakes them
+into an explicit grid (model/diver.py:56-74).
 """
 from __future__ import annotations
 
@@ -15,9 +16,9 @@ import torch.nn as nn
 from torch.autograd import Function
 
 from .. import calls
-from ..utils.integrator import integrate
+from ..utils.integrator import integrate, integrate_mlp
 from ..utils.ray_voxel_intersection import masked_intersect, ray_voxel_intersect
-from .mlps import PositionalEncoding, mlp
+from .mlps import ImplicitMLP, PositionalEncoding, mlp
 
 
 class _Composite(Function):
@@ -57,9 +58,12 @@ class DIVeR(nn.Module):
         self.xyzmin = -extent * 0.5
         self.xyzmax = extent * 0.5
         if getattr(hparams, 'implicit', False):
-            raise NotImplementedError('implicit grids are outside the explicit-grid render path')
-        voxels = torch.randn(*[self.voxel_num + 1] * 3, self.voxel_dim) * 1e-2
-        self.register_parameter('voxels', nn.Parameter(voxels, requires_grad=True))
+            self.mlp_voxel = ImplicitMLP(hparams.implicit_network_depth, hparams.implicit_channels,
+                                         hparams.implicit_skips, self.voxel_dim,
+                                         hparams.implicit_point_encode)
+        else:
+            voxels = torch.randn(*[self.voxel_num + 1] * 3, self.voxel_dim) * 1e-2
+            self.register_parameter('voxels', nn.Parameter(voxels, requires_grad=True))
         if getattr(hparams, 'fine', 0) != 0:
             m = int(self.voxel_num * self.mask_scale)
             self.register_parameter('voxel_mask', nn.Parameter(
@@ -72,12 +76,26 @@ class DIVeR(nn.Module):
         self.mlp2 = mlp(view_dim + mlp_out - 1, [mlp_dim] * mlp_depth, 3)
 
     def init_voxels(self, evaluate=True):
-        if evaluate:
-            raise NotImplementedError('initialising from the implicit MLP is outside this path')
+        """Bake the explicit grid [N+1]^3 x C: the implicit MLP at every corner (positions
+        (z, y, x) / ((N+1)/2) - 1, several z-slices per evaluation so each GEMM has ~2^18 rows),
+        or zeros (model/diver.py:56-74)."""
         n = self.voxel_num + 1
-        device = self.voxels.device
-        self.register_parameter('voxels', nn.Parameter(
-            torch.zeros(n, n, n, self.voxel_dim, device=device), requires_grad=True))
+        ref = self.mlp_voxel if hasattr(self, 'mlp_voxel') else self.voxels
+        device = next(ref.parameters()).device if isinstance(ref, nn.Module) else ref.device
+        if evaluate:
+            step = max(1, (1 << 18) // (n * n))
+            voxels = torch.empty(n, n, n, self.voxel_dim, device=device)
+            with torch.no_grad():
+                for z0 in range(0, n, step):
+                    zz, y, x = torch.meshgrid(torch.arange(z0, min(n, z0 + step), device=device),
+                                              torch.arange(n, device=device),
+                                              torch.arange(n, device=device), indexing='ij')
+                    p = torch.stack([zz, y, x], -1).float() / (n * 0.5) - 1.0
+                    voxels[z0:z0 + zz.shape[0]] = self.mlp_voxel(p.reshape(-1, 3)).reshape(
+                        zz.shape[0], n, n, self.voxel_dim)
+        else:
+            voxels = torch.zeros(n, n, n, self.voxel_dim, device=device)
+        self.register_parameter('voxels', nn.Parameter(voxels, requires_grad=True))
 
     def extract_features(self, os, ds):
         """os, ds [B, 3] -> (mask [B, M], features [K, C] in mask order, ts [B, M])."""
@@ -98,7 +116,12 @@ class DIVeR(nn.Module):
             coord_out = coord[:, 1:][mask]
         if not mask.any():
             return mask, None, None
-        return mask, integrate(self.voxels, coord_in.contiguous(), coord_out.contiguous()), ts
+        if hasattr(self, 'voxels'):
+            features = integrate(self.voxels, coord_in.contiguous(), coord_out.contiguous())
+        else:
+            features = integrate_mlp(self.mlp_voxel, self.voxel_num + 1, self.voxel_dim, coord_in,
+                                     coord_out)
+        return mask, features, ts
 
     def _decode_features(self, feature, ds, mask):
         B, M = mask.shape
@@ -113,7 +136,11 @@ class DIVeR(nn.Module):
         return color, sigma
 
     def decode(self, coord_in, coord_out, ds, mask):
-        feature = integrate(self.voxels, coord_in.contiguous(), coord_out.contiguous())
+        if hasattr(self, 'voxels'):
+            feature = integrate(self.voxels, coord_in.contiguous(), coord_out.contiguous())
+        else:
+            feature = integrate_mlp(self.mlp_voxel, self.voxel_num + 1, self.voxel_dim,
+                                    coord_in, coord_out)
         return self._decode_features(feature, ds, mask)
 
     def forward(self, os, ds):

@@ -91,6 +91,24 @@ int diver_integrate_backward(int n1, int channels, int64_t k, const float* coord
                              const float* coord_out, const float* grad_feature,
                              float* grad_voxels, diver_stream_t stream);
 
+/* Implicit grid (utils/integrator.py:3-60, integrate_mlp): the features of a voxel's corners come
+ * from an MLP evaluated once per distinct corner.  diver_corner_keys writes, per row, the keys
+ * z n1^2 + y n1 + x of the 8 corners of its integral (corner j = (j&1, (j>>1)&1, j>>2); upper
+ * corner clamped to n1 - 1 as integrator.py:20), keys [k][8] int64.  After the caller has
+ * evaluated the MLP at the distinct keys, diver_integrate_table integrates row r from
+ * table [U][channels] rows corner_row[8 r + j] with the same weights as diver_integrate;
+ * diver_integrate_table_backward adds d feature / d table ^T grad_feature into grad_table
+ * (atomic). */
+int diver_corner_keys(int n1, int64_t k, const float* coord_in, const float* coord_out,
+                      int64_t* keys, diver_stream_t stream);
+int diver_integrate_table(const float* table, int channels, int64_t k, const int64_t* corner_row,
+                          const float* coord_in, const float* coord_out, float* feature,
+                          diver_stream_t stream);
+int diver_integrate_table_backward(int channels, int64_t k, const int64_t* corner_row,
+                                   const float* coord_in, const float* coord_out,
+                                   const float* grad_feature, float* grad_table,
+                                   diver_stream_t stream);
+
 /* ---- compositing ------------------------------------------------------------------------------ */
 
 /* alpha = 1 - exp(-relu(sigma) * mask); weight = alpha * exclusive cumprod(1 - alpha);
