@@ -283,6 +283,8 @@ def integrate_table(table, corner_row, coord_in, coord_out):
         raise TypeError('corner_row must be int64')
     crow = corner_row.contiguous()
     K, C = ci.shape[0], table.shape[-1]
+    if table.dim() != 2 or tuple(crow.shape) != (K, 8) or tuple(co.shape) != (K, 3):
+        raise ValueError('table must be [U, C], corner_row [K, 8] and coordinates [K, 3]')
     out = torch.empty((K, C), dtype=torch.float32, device=dev)
     _lib.call('diver_integrate_table', table.data_ptr(), int(C), K, crow.data_ptr(), ci.data_ptr(),
               co.data_ptr(), out.data_ptr(), _s(ci))
@@ -295,6 +297,9 @@ def integrate_table_backward(n_rows_table, corner_row, coord_in, coord_out, grad
     ci, co = _f32(coord_in, 'coord_in'), _f32(coord_out, 'coord_out')
     g = _f32(grad_feature, 'grad_feature')
     K, C = g.shape
+    if corner_row.dtype != torch.int64 or tuple(corner_row.shape) != (K, 8) \
+            or tuple(ci.shape) != (K, 3) or tuple(co.shape) != (K, 3):
+        raise ValueError('corner_row must be int64 [K, 8] and coordinates [K, 3]')
     gt = torch.zeros((int(n_rows_table), C), dtype=torch.float32, device=dev)
     _lib.call('diver_integrate_table_backward', int(C), K, corner_row.contiguous().data_ptr(),
               ci.data_ptr(), co.data_ptr(), g.data_ptr(), gt.data_ptr(), _s(ci))
