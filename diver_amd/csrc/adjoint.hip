@@ -14,15 +14,17 @@
 
 namespace diver {
 
+// One wave per workgroup; lanes [0, rpw) take consecutive rays (rays_per_wave: small batches
+// spread over more waves).
 template <class D, bool RT>
-__global__ void __launch_bounds__(256) rows_kernel(RenderArgs a, const int32_t* __restrict__ count,
-                                                   const int64_t* __restrict__ row_off,
-                                                   float* __restrict__ rows) {
+__global__ void __launch_bounds__(64) rows_kernel(RenderArgs a, const int32_t* __restrict__ count,
+                                                  const int64_t* __restrict__ row_off,
+                                                  float* __restrict__ rows, int rpw) {
     static_assert(D::NPRE == 0, "rows are written for decoders without pre-layers");
     constexpr int C = D::C;
     constexpr int RW = 9 + C;
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= a.n_rays) return;
+    const int64_t r = (int64_t)blockIdx.x * rpw + threadIdx.x;
+    if ((int)threadIdx.x >= rpw || r >= a.n_rays) return;
     const int n = a.n;
     RayWalk ray;
     if constexpr (RT)
@@ -64,11 +66,12 @@ __global__ void __launch_bounds__(256) rows_kernel(RenderArgs a, const int32_t* 
 template <class D>
 static hipError_t launch_rows_d(const RenderArgs& a, const int32_t* count, const int64_t* off,
                                 float* rows, hipStream_t s) {
-    dim3 grid((unsigned)((a.n_rays + 255) / 256)), block(256);
+    const int rpw = rays_per_wave(a.n_rays);
+    dim3 grid((unsigned)((a.n_rays + rpw - 1) / rpw)), block(64);
     if (a.traversal == 1)
-        hipLaunchKernelGGL((rows_kernel<D, true>), grid, block, 0, s, a, count, off, rows);
+        hipLaunchKernelGGL((rows_kernel<D, true>), grid, block, 0, s, a, count, off, rows, rpw);
     else
-        hipLaunchKernelGGL((rows_kernel<D, false>), grid, block, 0, s, a, count, off, rows);
+        hipLaunchKernelGGL((rows_kernel<D, false>), grid, block, 0, s, a, count, off, rows, rpw);
     return hipGetLastError();
 }
 

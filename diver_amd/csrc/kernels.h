@@ -55,7 +55,21 @@ struct BatchArgs {
     ReqDesc req[kMaxRequests];
     int n_req;
     int64_t total;                 // ray ids over all requests
+    int rays_per_wave;             // ids per wavefront (64, or fewer for small launches; set by
+                                   // the launcher)
 };
+
+// Rays per wavefront for a launch of `total` rays: 64 when the launch fills the GPU, else halved
+// (down to 8) until it makes about 2048 waves.  A wave's time is set by its longest ray and by the
+// shading work of its own rays, so a small launch finishes sooner as more, lighter waves spread
+// over the 256 CUs.  A ray's result does not depend on the rays it shares a wave with.
+inline int rays_per_wave(int64_t total) {
+    constexpr int64_t kTargetWaves = 2048;
+    const int64_t want = total / kTargetWaves;
+    int rpw = 64;
+    while (rpw > 8 && rpw > want) rpw >>= 1;
+    return rpw;
+}
 
 hipError_t launch_ray_voxel_intersect(int64_t batch, int max_n, float xyzmin, float voxel_num,
                                       float voxel_size, const float* o, const float* v,

@@ -228,11 +228,14 @@ __global__ void __launch_bounds__(64 * kWavesPerGroup, tile_waves_per_simd<D>())
     acc[lane * 3 + 1] = 0.0f;
     acc[lane * 3 + 2] = 0.0f;
 
-    const int64_t g = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    // lanes [0, rays_per_wave) of each wave take consecutive ray ids; the others stay idle
+    const int rpw = p.rays_per_wave;
+    const int64_t g = ((int64_t)blockIdx.x * kWavesPerGroup + wid) * rpw + lane;
     int req = 0;
     int64_t out_idx = 0;
     float ox = 0.f, oy = 0.f, oz = 0.f, dx = 0.f, dy = 0.f, dz = 1.f;
-    const bool valid = g < p.total && ray_of(p, g, req, out_idx, ox, oy, oz, dx, dy, dz);
+    const bool valid = lane < rpw && g < p.total &&
+                       ray_of(p, g, req, out_idx, ox, oy, oz, dx, dy, dz);
     if (!valid) {
         ox = oy = oz = 0.f;
         dx = dy = 0.f;
@@ -428,7 +431,8 @@ bool tile_fits(int n) {
 template <class D, int PREC, bool RT>
 static hipError_t launch_t3(const BatchArgs& p, bool dch0, hipStream_t s) {
     constexpr int threads = 64 * kWavesPerGroup;
-    dim3 grid((unsigned)((p.total + threads - 1) / threads)), block(threads);
+    const int64_t per_group = (int64_t)p.rays_per_wave * kWavesPerGroup;
+    dim3 grid((unsigned)((p.total + per_group - 1) / per_group)), block(threads);
     size_t lds = render_tile_lds_bytes(p.g.n, D::H);
     if (dch0)
         hipLaunchKernelGGL((render_tile_kernel<D, PREC, RT, true>), grid, block, lds, s, p);
@@ -450,9 +454,11 @@ static hipError_t launch_t1(const BatchArgs& p, int prec, bool dch0, hipStream_t
                               : launch_t2<D, false>(p, prec, dch0, s);
 }
 
-hipError_t launch_render_tiles(int variant, const BatchArgs& p, int prec, bool dch0,
+hipError_t launch_render_tiles(int variant, const BatchArgs& args, int prec, bool dch0,
                                hipStream_t s) {
-    if (p.total == 0) return hipSuccess;
+    if (args.total == 0) return hipSuccess;
+    BatchArgs p = args;
+    p.rays_per_wave = rays_per_wave(p.total);
     switch (variant) {
         case 0: return launch_t1<DecB32>(p, prec, dch0, s);
         case 1: return launch_t1<DecB64>(p, prec, dch0, s);
