@@ -78,6 +78,7 @@ struct diver_decoder {
     float* W = nullptr;            // canonical fp32
     uint32_t* pack_bf16 = nullptr; // matrix-core A operands
     uint32_t* pack_f16 = nullptr;
+    float* pack_f32 = nullptr;     // FP32 matrix-core A operands (repacked on every update)
 };
 
 extern "C" {
@@ -274,6 +275,7 @@ int diver_decoder_create(const diver_decoder_desc* desc, const float* params_hos
         if (d->W) (void)hipFree(d->W);
         if (d->pack_bf16) (void)hipFree(d->pack_bf16);
         if (d->pack_f16) (void)hipFree(d->pack_f16);
+        if (d->pack_f32) (void)hipFree(d->pack_f32);
         delete d;
         return fail(code, "%s", what);
     };
@@ -302,6 +304,12 @@ int diver_decoder_create(const diver_decoder_desc* desc, const float* params_hos
             if (hipMemcpy(*dst, pk.data(), pk.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
                 return cleanup(DIVER_EHIP, "decoder pack upload failed");
         }
+        const int64_t nf = diver::pack_f32_size(C, H, desc->n_post);
+        if (hipMalloc(&d->pack_f32, sizeof(float) * nf) != hipSuccess)
+            return cleanup(DIVER_ENOMEM, "decoder pack allocation failed");
+        if (diver::launch_pack_f32(C, H, desc->n_post, d->W, d->pack_f32, nullptr) != hipSuccess ||
+            hipStreamSynchronize(nullptr) != hipSuccess)
+            return cleanup(DIVER_EHIP, "decoder FP32 pack failed");
     }
     *out = d;
     return DIVER_OK;
@@ -314,9 +322,14 @@ int diver_decoder_update(diver_decoder* d, const float* params, int64_t n_params
             (long long)d->n_params, (long long)n_params);
     d->packs_valid = false;
     d->density_ch0 = false;  // w_s / b_s may have changed: density from the full integral
-    return hip_status(hipMemcpyAsync(d->W, params, sizeof(float) * n_params,
-                                     hipMemcpyDeviceToDevice, S(stream)),
-                      "decoder_update");
+    int st = hip_status(hipMemcpyAsync(d->W, params, sizeof(float) * n_params,
+                                       hipMemcpyDeviceToDevice, S(stream)),
+                        "decoder_update");
+    if (st != DIVER_OK || !d->pack_f32) return st;
+    // the FP32 operands follow W on the same stream
+    return hip_status(diver::launch_pack_f32(d->desc.channels, d->desc.hidden, d->desc.n_post, d->W,
+                                             d->pack_f32, S(stream)),
+                      "decoder_update (FP32 pack)");
 }
 
 int diver_decoder_destroy(diver_decoder* d) {
@@ -327,6 +340,7 @@ int diver_decoder_destroy(diver_decoder* d) {
     if (d->W) (void)hipFree(d->W);
     if (d->pack_bf16) (void)hipFree(d->pack_bf16);
     if (d->pack_f16) (void)hipFree(d->pack_f16);
+    if (d->pack_f32) (void)hipFree(d->pack_f32);
     hipSetDevice(dev);
     delete d;
     return DIVER_OK;
@@ -453,7 +467,9 @@ static int render_common(const diver_grid* g, const diver_decoder* dec,
     a.xyzmin = g->xyzmin;
     a.voxel_size = g->voxel_size;
     a.W = dec->W;
-    a.pack = precision == DIVER_BF16 ? dec->pack_bf16 : dec->pack_f16;
+    a.pack = precision == DIVER_BF16   ? dec->pack_bf16
+             : precision == DIVER_FP16 ? dec->pack_f16
+                                       : reinterpret_cast<const uint32_t*>(dec->pack_f32);
     a.t_stop = opts->t_stop;
     a.alpha_skip = opts->alpha_skip;
     return DIVER_OK;

@@ -110,6 +110,10 @@ hipError_t launch_integrate_table_backward(int C, int64_t K, const int64_t* crow
 hipError_t launch_render_tiles(int variant, const BatchArgs& p, int prec, bool dch0,
                                hipStream_t s);
 bool tile_fits(int n);  // the grid's block bitmap fits the tile kernel's LDS
+// FP32 matrix-core A operands of a decoder without pre-layers (C channels, hidden H, npost post
+// layers), packed on the device from its canonical parameters W: pack_f32_size floats.
+int64_t pack_f32_size(int C, int H, int npost);
+hipError_t launch_pack_f32(int C, int H, int npost, const float* W, float* out, hipStream_t s);
 hipError_t launch_block_bits(const uint8_t* mask, int n, uint32_t* bits, hipStream_t s);
 hipError_t launch_brick_bits(const uint8_t* mask, int n, unsigned long long* bricks,
                              hipStream_t s);

@@ -60,6 +60,36 @@ __device__ __forceinline__ f32x16_t mfma32(u32x4_t a, u32x4_t b, f32x16_t c) {
     }
 }
 
+// FP32 products on the matrix cores: v_mfma_f32_32x32x2_f32 (FP32 inputs, FP32 accumulation).
+//   A (32 x 2): lane l holds A[l % 32][l / 32];   B (2 x 32): lane l holds B[l / 32][l % 32]
+//   D: the 32x32 layout above.
+// A hidden layer's D tiles feed the next layer's B operands in place when that layer's k steps
+// follow the D layout: step q = 16*mt + v pairs feature 32*mt + 8*(v/4) + (v%4) (k = 0, lanes
+// 0-31) with feature 32*mt + 8*(v/4) + 4 + (v%4) (k = 1, lanes 32-63), i.e. register v of D
+// tile mt is the B operand of step q.  The packed A operands follow the same order.
+__device__ __forceinline__ f32x16_t mfma32_f32(float a, float b, f32x16_t c) {
+    return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+}
+
+// Feature index of k step q, half h (0: lanes 0-31, 1: lanes 32-63) in the D-ordered k steps.
+__host__ __device__ constexpr int d_order_feature(int q, int h) {
+    return 32 * (q / 16) + 8 * ((q % 16) / 4) + 4 * h + (q % 4);
+}
+
+// Row vectors (K wide, one row per lane, fp32) -> FP32 B operands of both column tiles, one per
+// k pair (features 2kc, 2kc + 1): b0[kc] feeds tile 0 (rows of lanes 0-31), b1[kc] tile 1.
+template <int K>
+__device__ __forceinline__ void rows_to_b_f32(const float (&z)[K], float (&b0)[K / 2],
+                                              float (&b1)[K / 2]) {
+#pragma unroll
+    for (int kc = 0; kc < K / 2; ++kc) {
+        float x = z[2 * kc], y = z[2 * kc + 1];
+        swap32f(x, y);
+        b0[kc] = x;
+        b1[kc] = y;
+    }
+}
+
 // Row vectors (K wide, one row per lane, fp32) -> B operands of both column tiles, one per
 // 16-wide k chunk.  b0[kc] feeds tile 0, b1[kc] tile 1.
 template <int PREC, int K>

@@ -14,9 +14,9 @@
 //      wave's LDS queue.  The phase ends when >= 32 rows are queued or every lane is done.
 //   B. the wave takes up to 64 queued rows, one per lane: full FP32 segment integral of the
 //      row's voxel, then the colour decoder on the matrix cores, one 32-row column tile at a
-//      time (or FP32 per lane), and an LDS atomic add of weight * colour into the owner ray's
-//      accumulator.  The per-ray view term W_v PE(d) + b_c is computed once per ray in FP32 and
-//      kept in LDS.
+//      time (BF16/FP16 operands, or FP32 operands with v_mfma_f32_32x32x2_f32), and an LDS
+//      atomic add of weight * colour into the owner ray's accumulator.  The per-ray view term
+//      W_v PE(d) + b_c is computed once per ray in FP32 and kept in LDS.
 // Empty-space skipping: the workgroup keeps the 8^3-block occupancy bitmap in LDS and each lane
 // caches the bit-packed mask of its current 4^3 brick; a voxel in an empty block never reads
 // memory.  Skipping changes no arithmetic of the walk, so the intervals are bit-identical to the
@@ -150,6 +150,96 @@ __device__ __forceinline__ void shade_tile(const RenderArgs& a, const float* que
 #pragma unroll
     for (int kc = 0; kc < KCH; ++kc) o = mfma32<PREC>(pk[(pofs + kc) * 64 + lane], e[kc], o);
     // output rows 0..2 sit in registers 0..2 of lanes 0..31, column = row of the tile
+    if (lane < 32 && live) {
+        const float* bo = a.W + D::out_off + 3 * H;
+        const float wgt = queue[row * kRowFloats + 6];
+        atomicAdd(&acc[owner * 3 + 0], wgt * sigmoidf(o[0] + bo[0]));
+        atomicAdd(&acc[owner * 3 + 1], wgt * sigmoidf(o[1] + bo[1]));
+        atomicAdd(&acc[owner * 3 + 2], wgt * sigmoidf(o[2] + bo[2]));
+    }
+}
+
+// acc += sum over k steps q < K of A(step q) B(bv[q]) with FP32 operands; the A operands (64
+// floats per step at pk) are loaded 8 steps at a time, the next 8 in flight during the current
+// products, which bounds the registers they hold.
+template <int K>
+__device__ __forceinline__ f32x16_t mfma_chain_f32(const float* __restrict__ pk, int lane,
+                                                   const float (&bv)[K], f32x16_t acc) {
+    static_assert(K % 8 == 0, "k steps in chunks of 8");
+    float w[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) w[i] = pk[i * 64 + lane];
+#pragma unroll
+    for (int k0 = 0; k0 < K; k0 += 8) {
+        float wn[8];
+        if (k0 + 8 < K) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) wn[i] = pk[(k0 + 8 + i) * 64 + lane];
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) acc = mfma32_f32(w[i], bv[k0 + i], acc);
+        __builtin_amdgcn_sched_barrier(0);
+        if (k0 + 8 < K) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) w[i] = wn[i];
+        }
+    }
+    return acc;
+}
+
+// shade_tile with FP32 operands (v_mfma_f32_32x32x2_f32).  Packed A operands (floats, 64 per k
+// step): head W_c in natural k pairs (2kc, 2kc + 1) for (mt, kc); post and output layers in the
+// D order of d_order_feature for (mt, q), so relu(D register v of tile mt) is the B operand of
+// step 16 mt + v.  The output layer is padded to 32 rows.
+template <class D>
+__device__ __forceinline__ void shade_tile_f32(const RenderArgs& a, const float* queue,
+                                               const float* hvs, float* acc, int t, int nbatch,
+                                               const float (&b)[D::Z / 2], int lane) {
+    constexpr int H = D::H, Z = D::Z;
+    constexpr int MT = H / 32, KH = Z / 2, KP = H / 2;
+    const float* pk = reinterpret_cast<const float*>(a.pack);
+    // The operands are the same for every batch; an opaque pointer keeps their loads inside the
+    // batch (held across the walk they would take 128 registers at hidden width 64).
+    asm volatile("" : "+s"(pk));
+    const int c = lane & 31;
+    const int half = lane & 32;
+    const int row = 32 * t + c;
+    const bool live = row < nbatch;
+    const int owner = live ? __builtin_bit_cast(int, queue[row * kRowFloats + 7]) : 0;
+    f32x16_t y[MT];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            float4 h4 = *reinterpret_cast<const float4*>(hvs + owner * H + 32 * mt + 8 * g + (half >> 3));
+            y[mt][4 * g + 0] = h4.x;
+            y[mt][4 * g + 1] = h4.y;
+            y[mt][4 * g + 2] = h4.z;
+            y[mt][4 * g + 3] = h4.w;
+        }
+        y[mt] = mfma_chain_f32<KH>(pk + mt * KH * 64, lane, b, y[mt]);
+    }
+    int pofs = MT * KH * 64;
+    static_assert(D::NPOST <= 1, "one post layer at most");
+    if constexpr (D::NPOST == 1) {
+        float e[KP];
+#pragma unroll
+        for (int q = 0; q < KP; ++q) e[q] = fmaxf(y[q / 16][q % 16], 0.0f);
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+            f32x16_t s;
+#pragma unroll
+            for (int i = 0; i < 16; ++i)
+                s[i] = a.W[D::post_off(0) + H * H + 32 * mt + 8 * (i >> 2) + (half >> 3) + (i & 3)];
+            y[mt] = mfma_chain_f32<KP>(pk + pofs + mt * KP * 64, lane, e, s);
+        }
+        pofs += MT * KP * 64;
+    }
+    float e[KP];
+#pragma unroll
+    for (int q = 0; q < KP; ++q) e[q] = fmaxf(y[q / 16][q % 16], 0.0f);
+    f32x16_t o = {};
+    o = mfma_chain_f32<KP>(pk + pofs, lane, e, o);
     if (lane < 32 && live) {
         const float* bo = a.W + D::out_off + 3 * H;
         const float wgt = queue[row * kRowFloats + 6];
@@ -360,18 +450,10 @@ __global__ void __launch_bounds__(64 * kWavesPerGroup, tile_waves_per_simd<D>())
 #pragma unroll
                 for (int i = 0; i < Z; ++i) z[i] = 0.0f;
             }
-            if (mine) {
-                const int owner = __builtin_bit_cast(int, queue[lane * kRowFloats + 7]);
-                float hvr[H];
-#pragma unroll
-                for (int j = 0; j < H; ++j) hvr[j] = hvs[owner * H + j];
-                float rgb[3];
-                colour<D>(a.W, z, hvr, rgb);
-                const float wgt = queue[lane * kRowFloats + 6];
-                atomicAdd(&acc[owner * 3 + 0], wgt * rgb[0]);
-                atomicAdd(&acc[owner * 3 + 1], wgt * rgb[1]);
-                atomicAdd(&acc[owner * 3 + 2], wgt * rgb[2]);
-            }
+            float b0[Z / 2], b1[Z / 2];
+            rows_to_b_f32<Z>(z, b0, b1);
+            shade_tile_f32<D>(a, queue, hvs, acc, 0, nbatch, b0, lane);
+            if (nbatch > 32) shade_tile_f32<D>(a, queue, hvs, acc, 1, nbatch, b1, lane);
         } else {
             static_assert(Z == C, "the packed gather feeds the head input directly");
             uint32_t zp[Z / 2];
@@ -414,6 +496,48 @@ __global__ void __launch_bounds__(64 * kWavesPerGroup, tile_waves_per_simd<D>())
             make_float4(acc[lane * 3], acc[lane * 3 + 1], acc[lane * 3 + 2], T);
         if (rq.count) rq.count[out_idx] = used;
     }
+}
+
+// FP32 matrix-core A operands (shade_tile_f32's order) from the canonical parameters of a decoder
+// without pre-layers: one float per (k step, lane).
+__global__ void __launch_bounds__(256) pack_f32_kernel(int C, int H, int npost,
+                                                       const float* __restrict__ W,
+                                                       float* __restrict__ out, int64_t total) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    const int r = (int)(idx & 31), h = (int)((idx >> 5) & 1);
+    int64_t step = idx >> 6;
+    const int64_t wc = C + 1;  // w_s[C], b_s, then W_c[H][C]
+    const int64_t post0 = wc + (int64_t)H * C + (int64_t)H * kPE + H;
+    const int64_t outo = post0 + (int64_t)npost * (H * H + H);
+    const int64_t head_steps = (int64_t)(H / 32) * (C / 2);
+    const int64_t post_steps = (int64_t)(H / 32) * (H / 2);
+    float v;
+    if (step < head_steps) {
+        const int mt = (int)(step / (C / 2)), kc = (int)(step % (C / 2));
+        v = W[wc + (int64_t)(32 * mt + r) * C + 2 * kc + h];
+    } else if ((step -= head_steps) < npost * post_steps) {
+        const int j = (int)(step / post_steps);
+        const int64_t rem = step % post_steps;
+        const int mt = (int)(rem / (H / 2)), q = (int)(rem % (H / 2));
+        v = W[post0 + (int64_t)j * (H * H + H) + (int64_t)(32 * mt + r) * H + d_order_feature(q, h)];
+    } else {
+        const int q = (int)(step - npost * post_steps);
+        v = r < 3 ? W[outo + (int64_t)r * H + d_order_feature(q, h)] : 0.0f;
+    }
+    out[idx] = v;
+}
+
+int64_t pack_f32_size(int C, int H, int npost) {
+    return (int64_t)H * C + (int64_t)npost * H * H + 32 * (int64_t)H;
+}
+
+hipError_t launch_pack_f32(int C, int H, int npost, const float* W, float* out, hipStream_t s) {
+    if (C % 2 || H % 32 || C <= 0) return hipErrorInvalidValue;
+    const int64_t total = pack_f32_size(C, H, npost);
+    hipLaunchKernelGGL(pack_f32_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, C, H,
+                       npost, W, out, total);
+    return hipGetLastError();
 }
 
 static size_t render_tile_lds_bytes(int n, int H) {
