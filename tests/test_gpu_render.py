@@ -130,3 +130,28 @@ def test_render_256_subsample(cameras, dev):
         p = psnr(rgb.cpu().numpy(), rgb_ref)
         assert p >= min_psnr, (precision, p)
         assert _count_ok(cnt.cpu().numpy(), ref['used']) >= 0.999
+
+
+@pytest.mark.parametrize('hidden', [32, 64])
+def test_fp32_operands_follow_device_update(scene64, cameras, dev, hidden):
+    """The FP32 decoder runs on the matrix cores from operands packed on the device, and
+    diver_decoder_update (how training pushes new weights) repacks them: a decoder updated to new
+    parameters must render as one created from those parameters.  Operands left from the old
+    weights would be off by ~1e-2; the bound 1e-6 leaves room only for the density's
+    full-integral form that an updated decoder takes instead of the channel-0 shortcut."""
+    from diver_amd import calls
+    sc = scene64
+    grid = SceneGrid.from_synthetic(sc, dev)
+    layers = S.baseline_decoder_layers(32, hidden)
+    dec = Decoder(from_baseline(layers, 32))
+    new = from_baseline([(w * 0.8, b + 0.05) for w, b in layers], 32)
+    fresh = Decoder(new)
+    before = [render_camera(grid, dec, cam, 64, 64, precision='fp32')[0]
+              for cam in (cameras[0], cameras[11])]
+    calls.decoder_update(dec, torch.from_numpy(np.ascontiguousarray(new.params)).to(dev))
+    for cam, rgb0 in zip((cameras[0], cameras[11]), before):
+        rgb_u, T_u, _ = render_camera(grid, dec, cam, 64, 64, precision='fp32')
+        rgb_f, T_f, _ = render_camera(grid, fresh, cam, 64, 64, precision='fp32')
+        assert float((rgb_u - rgb0).abs().max()) > 1e-3  # the update changed the image
+        assert float((rgb_u - rgb_f).abs().max()) <= 1e-6
+        assert float((T_u - T_f).abs().max()) <= 1e-6
