@@ -5,7 +5,9 @@ requests (scene, camera, resolution) from any thread; a render loop thread packs
 requests of one scene (FIFO, up to ``max_requests`` requests and ``max_rays`` rays) into one
 ``diver_render_batch`` launch and keeps up to ``n_streams`` launches in flight.  Requests that
 arrive while launches run are admitted into the next launch as soon as a stream slot and memory
-are free, so their work overlaps the frames already on the GPU.
+are free, so their work overlaps the frames already on the GPU.  A second thread waits on the
+oldest launch's completion event (blocking, without polling) and releases its frames and its
+stream slot the moment the GPU finishes it.
 
 Memory admission: every resident scene and every in-flight output buffer is charged against
 ``hbm_budget`` bytes.  A scene that does not fit is refused at ``add_scene``; a request whose
@@ -122,7 +124,9 @@ class RenderServer:
         self.launches = 0
         self.completed: List[FrameRequest] = []
         self._thread = threading.Thread(target=self._loop, name='diver-render-loop', daemon=True)
+        self._retirer = threading.Thread(target=self._retire_loop, name='diver-retire', daemon=True)
         self._thread.start()
+        self._retirer.start()
 
     # ---- scenes ---------------------------------------------------------------------------
     def add_scene(self, name: str, grid: SceneGrid, decoder: Decoder, precision: str = 'bf16',
@@ -154,14 +158,15 @@ class RenderServer:
             if req.rays > self.max_rays:
                 raise AdmissionError(f'{req.rays} rays exceed the per-launch limit {self.max_rays}')
             self._pending.append(req)
-            self._cv.notify()
+            self._cv.notify_all()
         return req
 
     def close(self):
         with self._cv:
             self._stop = True
-            self._cv.notify()
+            self._cv.notify_all()
         self._thread.join()
+        self._retirer.join()
 
     def __enter__(self):
         return self
@@ -191,18 +196,9 @@ class RenderServer:
             self._pending.remove(req)
         return batch
 
-    def _retire(self):
-        done = [launch for launch in self._inflight if launch.event.query()]
-        if not done:
-            return
-        now = time.perf_counter()
-        for launch in done:
-            self._inflight.remove(launch)
-            self._inflight_bytes -= launch.nbytes
-            for req in launch.requests:
-                req.t_done = now
-                self.completed.append(req)
-                req._done.set()
+    def _finished(self):
+        """Nothing pending or in flight after close().  Called with the lock held."""
+        return self._stop and not self._pending and not self._inflight
 
     def _launch(self, batch: List[FrameRequest]):
         sc = self.scenes[batch[0].scene]
@@ -231,26 +227,41 @@ class RenderServer:
         self._inflight.append(_Launch(ev, batch, nbytes))
         self._inflight_bytes += nbytes
         self.launches += 1
+        self._cv.notify_all()  # the retire thread waits for a launch to watch
 
     def _loop(self):
+        """Launch thread: packs admissible requests whenever a stream slot is free."""
+        torch.cuda.set_device(self.device)
+        with self._cv:
+            while not self._finished():
+                batch = self._take_batch() if len(self._inflight) < len(self.streams) else []
+                if batch:
+                    self._launch(batch)
+                else:
+                    # woken by an arrival, a completed launch (slot and memory freed) or close()
+                    self._cv.wait()
+
+    def _retire_loop(self):
+        """Retire thread: blocks on the oldest launch's event (the GIL is released while it
+        waits), then hands its frames to their clients and frees its slot and memory."""
         torch.cuda.set_device(self.device)
         while True:
             with self._cv:
-                self._retire()
-                if self._stop and not self._pending and not self._inflight:
-                    return
-                if len(self._inflight) < len(self.streams):
-                    batch = self._take_batch()
-                    if batch:
-                        self._launch(batch)
-                        continue
-                    # a stream is free but nothing is admissible: wake on an arrival, or poll
-                    # the launches in flight for completions
-                    self._cv.wait(timeout=2e-4 if self._inflight else 1e-2)
-                    continue
-                oldest = self._inflight[0].event
-            # every stream busy: wait for the oldest launch
-            oldest.synchronize()
+                while not self._inflight:
+                    if self._finished():
+                        return
+                    self._cv.wait()
+                launch = self._inflight[0]
+            launch.event.synchronize()
+            now = time.perf_counter()
+            with self._cv:
+                self._inflight.remove(launch)
+                self._inflight_bytes -= launch.nbytes
+                for req in launch.requests:
+                    req.t_done = now
+                    self.completed.append(req)
+                    req._done.set()
+                self._cv.notify_all()
 
     # ---- statistics -----------------------------------------------------------------------
     def stats(self, since: float = 0.0) -> dict:
