@@ -37,9 +37,15 @@ __global__ void __launch_bounds__(64) rows_kernel(RenderArgs a, const int32_t* _
     float* out = rows + row_off[r] * RW;
     Kahan S;
     float T = 1.0f;
+    // occupancy through the block/brick bits when the grid has them (the forward's lookup)
+    const bool cached = a.blocks != nullptr && a.bricks != nullptr;
+    OccCache occ;
     for (int k = 0; k < todo; ++k) {
         float px, py, pz, qx, qy, qz;
-        if (!ray.next<RT>(a.mask, n, px, py, pz, qx, qy, qz)) break;
+        const bool hit = cached ? next_cached<RT>(ray, occ, a.mask, a.blocks, a.bricks, n, px, py,
+                                                  pz, qx, qy, qz)
+                                : ray.next<RT>(a.mask, n, px, py, pz, qx, qy, qz);
+        if (!hit) break;
         float w[8];
         int64_t off[8];
         segment_corners<C>(px, py, pz, qx, qy, qz, n, w, off);

@@ -191,6 +191,35 @@ struct OccCache {
     }
 };
 
+// RayWalk::next with the occupancy read through an OccCache (8^3-block bits and 4^3-brick words
+// instead of one mask byte per voxel): the same steps, the same intervals.
+template <bool OPEN = false>
+__device__ __forceinline__ bool next_cached(RayWalk& ray, OccCache& occ,
+                                            const uint8_t* __restrict__ mask,
+                                            const uint32_t* __restrict__ blocks,
+                                            const unsigned long long* __restrict__ bricks, int n,
+                                            float& px, float& py, float& pz, float& qx, float& qy,
+                                            float& qz) {
+    const float nf = (float)n;
+    while (ray.alive && ray.steps < ray.max_steps) {
+        ++ray.steps;
+        float nx, ny, nz;
+        ray.step(ray.lx, ray.ly, ray.lz, nx, ny, nz);
+        if (OPEN ? outside_open(nx, ny, nz, nf) : outside(nx, ny, nz, nf)) break;
+        int vx, vy, vz;
+        segment_voxel1(ray.lx, ray.ly, ray.lz, nx, ny, nz, vx, vy, vz);
+        const bool occd = occ.lookup(mask, blocks, bricks, n, vx, vy, vz);
+        px = ray.lx; py = ray.ly; pz = ray.lz;
+        ray.lx = nx; ray.ly = ny; ray.lz = nz;
+        if (occd) {
+            qx = nx; qy = ny; qz = nz;
+            return true;
+        }
+    }
+    ray.alive = false;
+    return false;
+}
+
 // Corner weights of the segment mean of the trilinear field (integrator.py:92-111 arranged as
 // mlp_eval.cu:83-101) and the 8 corner offsets into a [(n+1)^3][C] grid.  I is the offset type:
 // uint32_t when the caller has checked (n+1)^3 * C < 2^32 (cheaper index arithmetic), else
