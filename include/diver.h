@@ -149,8 +149,9 @@ int diver_decoder_create(const diver_decoder_desc* desc, const float* params_hos
                          int64_t n_params, diver_decoder** out);
 int diver_decoder_destroy(diver_decoder* dec);
 /* Replace the parameters from device memory (asynchronous copy on `stream`, e.g. after an
- * optimizer step).  Only the FP32 decoder path may be used afterwards (the BF16/FP16 operands
- * are derived at creation). */
+ * optimizer step).  The FP32 matrix-core operands are repacked on the device on the same stream;
+ * only the FP32 decoder path may be used afterwards (the BF16/FP16 operands are derived on the
+ * host at creation). */
 int diver_decoder_update(diver_decoder* dec, const float* params, int64_t n_params,
                          diver_stream_t stream);
 int64_t diver_decoder_param_count(const diver_decoder_desc* desc);
