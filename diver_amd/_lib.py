@@ -21,6 +21,7 @@ EXPORTS = [
     'diver_camera_rays', 'diver_block_bits', 'diver_walk_rays', 'diver_render_rows',
     'diver_decoder_update', 'diver_brick_bits', 'diver_render_batch', 'diver_alpha_map',
     'diver_corner_keys', 'diver_integrate_table', 'diver_integrate_table_backward',
+    'diver_adam_step',
 ]
 
 MAX_REQUESTS = 16
@@ -97,6 +98,8 @@ def _declare(L):
         'diver_corner_keys': [i32, i64, vp, vp, vp, vp],
         'diver_integrate_table': [vp, i32, i64, vp, vp, vp, vp, vp],
         'diver_integrate_table_backward': [i32, i64, vp, vp, vp, vp, vp, vp],
+        'diver_adam_step': [vp, vp, vp, vp, i64, ctypes.c_double, ctypes.c_double, fl, fl, fl, fl,
+                            i32, vp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

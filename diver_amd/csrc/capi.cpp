@@ -424,6 +424,24 @@ int diver_brick_bits(const uint8_t* mask, int n, uint64_t* bricks, diver_stream_
                       "brick_bits");
 }
 
+int diver_adam_step(float* param, float* grad, float* exp_avg, float* exp_avg_sq, int64_t n,
+                    double beta1, double beta2, float eps, float weight_decay, float step_size,
+                    float bc2_sqrt, int clear_grad, diver_stream_t stream) {
+    REQUIRE(n >= 0, "n < 0");
+    if (n == 0) return DIVER_OK;
+    REQUIRE(param && grad && exp_avg && exp_avg_sq, "null pointer");
+    const uintptr_t al = reinterpret_cast<uintptr_t>(param) | reinterpret_cast<uintptr_t>(grad) |
+                         reinterpret_cast<uintptr_t>(exp_avg) |
+                         reinterpret_cast<uintptr_t>(exp_avg_sq);
+    REQUIRE(al % 4 == 0, "parameter, gradient and moment arrays must be float-aligned");
+    REQUIRE(beta1 >= 0.0 && beta1 < 1.0 && beta2 >= 0.0 && beta2 < 1.0, "betas outside [0, 1)");
+    REQUIRE(eps >= 0.0f && bc2_sqrt > 0.0f && std::isfinite(step_size), "bad Adam scalars");
+    return hip_status(diver::launch_adam(param, grad, exp_avg, exp_avg_sq, n, beta1, beta2, eps,
+                                         weight_decay, step_size, bc2_sqrt, clear_grad != 0,
+                                         S(stream)),
+                      "adam_step");
+}
+
 int diver_block_bits(const uint8_t* mask, int n, uint32_t* bits, diver_stream_t stream) {
     REQUIRE(mask && bits, "null pointer");
     REQUIRE(n >= 8 && n % 8 == 0, "n must be a positive multiple of 8");

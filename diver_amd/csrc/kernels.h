@@ -115,6 +115,9 @@ bool tile_fits(int n);  // the grid's block bitmap fits the tile kernel's LDS
 int64_t pack_f32_size(int C, int H, int npost);
 hipError_t launch_pack_f32(int C, int H, int npost, const float* W, float* out, hipStream_t s);
 hipError_t launch_block_bits(const uint8_t* mask, int n, uint32_t* bits, hipStream_t s);
+hipError_t launch_adam(float* p, float* g, float* m, float* v, int64_t n, double b1, double b2,
+                       float eps, float wd, float step_size, float bc2_sqrt, bool clear,
+                       hipStream_t st);
 hipError_t launch_brick_bits(const uint8_t* mask, int n, unsigned long long* bricks,
                              hipStream_t s);
 hipError_t launch_walk(const RenderArgs& a, int max_n, float* isect, int32_t* num, hipStream_t s);

@@ -16,7 +16,8 @@ training forward, model/diver.py:183-207).
              drop-in ops, keeping every interval's coordinates, features and activations, as the
              reference's model/diver.py:141-207 does.
 
-Both paths use Adam's fused single-pass update.  Prints one JSON line per run: steps/s, the
+Both paths use the same Adam: by default diver_amd.optim.Adam (one HIP pass per parameter, which
+also clears the gradient), or torch's fused Adam with --optimizer torch.  Prints one JSON line per run: steps/s, the
 forward + backward and Adam milliseconds per step (HIP events on the compute stream), and the
 forward + backward device-memory high-water mark above the resident state (parameters, gradients,
 optimizer moments) for each path.
@@ -36,6 +37,7 @@ from . import synthetic as S
 from .adjoint import BenchmarkDecoder, FusedRenderer, positional_encoding
 from .decoder import Decoder, from_baseline
 from .model.diver import composite
+from .optim import Adam as DiverAdam
 from .renderer import SceneGrid
 from .utils.integrator import integrate
 from .utils.ray_voxel_intersection import masked_intersect
@@ -61,7 +63,7 @@ def stored_forward(voxels, dec, mask, n, o, d):
     return rgb
 
 
-def run(path, steps, warmup, n_rays, dev):
+def run(path, steps, warmup, n_rays, dev, optimizer='diver'):
     sc = S.config2_scene(32, 256)
     gt = S.config2_scene(32, 256, seed=S.SEED_GT)
     cams = S.hemisphere_cameras()
@@ -91,14 +93,18 @@ def run(path, steps, warmup, n_rays, dev):
         mask = torch.from_numpy(sc.mask).to(dev)
         params = [voxels] + list(dec.parameters())
         fwd = lambda o, d: stored_forward(voxels, dec, mask, 256, o, d)
-    # one fused pass over each parameter (torch's multi-pass foreach Adam takes ~2.3x longer on
-    # the 2.17 GB grid); same update rule
-    opt = torch.optim.Adam(params, lr=1e-3, fused=True)
+    if optimizer == 'diver':
+        # one HIP pass per parameter that also clears its gradient (no zero_grad memset)
+        opt = DiverAdam(params, lr=1e-3, clear_grad=True)
+    else:
+        # torch's fused update (its default multi-pass foreach Adam takes ~2.3x longer on the grid)
+        opt = torch.optim.Adam(params, lr=1e-3, fused=True)
     ev = []
 
     def step(b, track=None, timed=False):
         og, dg, tg = b
-        opt.zero_grad(set_to_none=False)
+        if optimizer != 'diver':
+            opt.zero_grad(set_to_none=False)
         if track is not None:
             torch.cuda.synchronize()
             torch.cuda.reset_peak_memory_stats(dev)
@@ -148,15 +154,17 @@ def main(argv=None):
     ap.add_argument('--warmup', type=int, default=3)
     ap.add_argument('--rays', type=int, default=8192)
     ap.add_argument('--path', choices=['both', 'recompute', 'stored'], default='both')
+    ap.add_argument('--optimizer', choices=['diver', 'torch'], default='diver',
+                    help="diver: one-pass HIP Adam (diver_adam_step); torch: torch's fused Adam")
     args = ap.parse_args(argv)
     dev = torch.device('cuda', 0)
     out = {'workload': 'training step, 256^3 x 32 trainable grid, hidden-64 benchmark decoder, '
                        f'{args.rays} random rays, MSE + Adam, no early termination',
-           'steps': args.steps}
+           'steps': args.steps, 'optimizer': args.optimizer}
     paths = ('recompute', 'stored') if args.path == 'both' else (args.path,)
     for path in paths:
         print(f'[train] {path}', file=sys.stderr, flush=True)
-        out[path] = run(path, args.steps, args.warmup, args.rays, dev)
+        out[path] = run(path, args.steps, args.warmup, args.rays, dev, args.optimizer)
         torch.cuda.empty_cache()
     if len(paths) == 2:
         out['speedup_recompute_vs_stored'] = (out['recompute']['steps_per_s'] /

@@ -281,6 +281,19 @@ int diver_block_bits(const uint8_t* mask, int n, uint32_t* bits, diver_stream_t 
  * No counterpart in the reference (an acceleration structure for diver_render_*). */
 int diver_brick_bits(const uint8_t* mask, int n, uint64_t* bricks, diver_stream_t stream);
 
+/* One Adam step over n float32 parameters in device memory, replacing the update of
+ * torch.optim.Adam(params, lr, weight_decay) that the reference trains with (train.py:39-47):
+ *   g' = g + weight_decay * p;  m = beta1 m + (1 - beta1) g';  v = beta2 v + (1 - beta2) g'^2;
+ *   p -= step_size * m / (sqrt(v) / bc2_sqrt + eps)
+ * with step_size = lr / (1 - beta1^t) and bc2_sqrt = sqrt(1 - beta2^t) for step t, computed by the
+ * caller.  The betas are doubles, as torch's are: 1 - beta is rounded to FP32 once from them.
+ * clear_grad != 0 also sets the gradient to zero in the same pass.  Arrays that are all 16-byte
+ * aligned are processed in float4 groups, others element by element.  One read of p, g, m, v and
+ * one write of p, m, v (and g) per parameter. */
+int diver_adam_step(float* param, float* grad, float* exp_avg, float* exp_avg_sq, int64_t n,
+                    double beta1, double beta2, float eps, float weight_decay, float step_size,
+                    float bc2_sqrt, int clear_grad, diver_stream_t stream);
+
 /* The occupied intervals the fused renderer composites, one walk per ray (diver_traversal):
  * the offline walk equals masked_intersect (utils/ray_voxel_intersection/src/
  * ray_voxel_intersect.cu:192-319); the real-time walk equals aabb_intersect followed by ray_march

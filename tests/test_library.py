@@ -45,6 +45,21 @@ def test_version_and_validation_without_gpu():
         _lib.call('diver_integrate', None, 5, 6, 3, None, None, None, None)
 
 
+def test_adam_validation_without_gpu():
+    from diver_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip('libdiver.so not built')
+    L = _lib.lib()
+    args = (0.9, 0.999, 1e-8, 0.0, 1e-3, 1.0, 0, None)
+    assert L.diver_adam_step(None, None, None, None, 0, *args) == 0  # nothing to do
+    st = L.diver_adam_step(None, 16, 16, 16, 4, *args)
+    assert st == 1 and b'null' in L.diver_last_error()
+    st = L.diver_adam_step(16, 18, 16, 16, 4, *args)
+    assert st == 1 and b'aligned' in L.diver_last_error()
+    st = L.diver_adam_step(16, 16, 16, 16, 4, 1.5, *args[1:])
+    assert st == 1 and b'betas' in L.diver_last_error()
+
+
 def test_decoder_validation_without_gpu():
     import ctypes as C
     import numpy as np
