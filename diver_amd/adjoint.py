@@ -46,6 +46,22 @@ def split_canonical(p: torch.Tensor, C: int, H: int):
     return (ws, bs, wc.view(H, C), wv.view(H, PE_DIM), bc, w2.view(H, H), b2, wo.view(3, H), bo)
 
 
+def _wgrad(a: torch.Tensor, b: torch.Tensor, chunk: int = 4096) -> torch.Tensor:
+    """a.T @ b for tall a [K, m] and b [K, n] (a weight gradient: a reduction over all K rows into
+    at most 64 x 64).  One GEMM of that shape has only a handful of output tiles to spread over the
+    GPU, so the K reduction is split into chunks of `chunk` rows: one batched product of the chunks,
+    then a sum over them."""
+    K = a.shape[0]
+    if K <= 2 * chunk:
+        return a.T @ b
+    S = (K + chunk - 1) // chunk
+    pad = S * chunk - K
+    a = torch.nn.functional.pad(a, (0, 0, 0, pad)) if pad else a
+    b = torch.nn.functional.pad(b, (0, 0, 0, pad)) if pad else b
+    at = a.reshape(S, chunk, a.shape[1]).transpose(1, 2).contiguous()
+    return torch.bmm(at, b.reshape(S, chunk, b.shape[1]).contiguous()).sum(0)
+
+
 class _FusedRender(Function):
     @staticmethod
     def forward(ctx, voxels, params, grid, decoder, o, d, t_stop, white_back, traversal,
@@ -104,17 +120,17 @@ class _FusedRender(Function):
         ds = dsigma * (torch.sigmoid(s) if act == 1 else (s > 0).to(s.dtype))
         dc = gk * w[:, None]
         do = dc * c * (1.0 - c)
-        dWo = do.T @ r2
+        dWo = _wgrad(do, r2)
         dbo = do.sum(0)
         dr2 = (do @ Wo) * (y2 > 0)
-        dW2 = dr2.T @ r1
+        dW2 = _wgrad(dr2, r1)
         db2 = dr2.sum(0)
         dr1 = (dr2 @ W2) * (y1 > 0)
-        dWc = dr1.T @ F
-        dWv = dr1.T @ pe
+        dWc = _wgrad(dr1, F)
+        dWv = _wgrad(dr1, pe)
         dbc = dr1.sum(0)
         dF = dr1 @ Wc + ds[:, None] * ws[None, :]
-        dws = ds @ F
+        dws = _wgrad(ds[:, None], F)[0]
         dbs = ds.sum().reshape(1)
         gp = torch.cat([dws, dbs, dWc.reshape(-1), dWv.reshape(-1), dbc, dW2.reshape(-1), db2,
                         dWo.reshape(-1), dbo])
