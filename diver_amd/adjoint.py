@@ -66,7 +66,9 @@ class _FusedRender(Function):
     @staticmethod
     def forward(ctx, voxels, params, grid, decoder, o, d, t_stop, white_back, traversal,
                 accumulate):
-        calls.decoder_update(decoder, params)
+        # the density row (w_s = e_0, b_s = 0: softplus(F[0])) is constant, so only the colour
+        # part is pushed and the kernel keeps its channel-0 density path
+        calls.decoder_update(decoder, params, first=decoder.channels + 1)
         opts = calls.render_opts('fp32', traversal, t_stop, 0.0)
         rgba, count = calls.render_rays(grid, decoder, o, d, opts)
         ctx.save_for_backward(params, o, d, rgba, count)
@@ -90,7 +92,7 @@ class _FusedRender(Function):
         if K == 0:
             gv0 = None if ctx.accumulate is not None else torch.zeros(ctx.vshape, device=dev)
             return gv0, torch.zeros_like(params), *([None] * 8)
-        calls.decoder_update(decoder, params)
+        calls.decoder_update(decoder, params, first=decoder.channels + 1)
         rows = calls.render_rows(grid, decoder, o, d, ctx.opts, count, row_off, K)
         ray = torch.repeat_interleave(torch.arange(R, device=dev), cnt64)
         pin, pout = rows[:, 0:3], rows[:, 3:6]

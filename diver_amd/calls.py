@@ -244,11 +244,18 @@ def render_rows(grid, decoder, o, d, opts, count, row_offset, n_rows):
     return rows
 
 
-def decoder_update(decoder, params):
-    """Copy canonical parameters (device float32 tensor) into the decoder handle."""
+def decoder_update(decoder, params, first: int = 0):
+    """Copy canonical parameters (device float32 tensor, the whole canonical vector) into the
+    decoder handle.  With first > 0 only params[first:] is copied (diver_decoder_update_range);
+    first >= channels + 1 leaves the density row, and with it the channel-0 density path, alone."""
     _dev(params)
     p = _f32(params.detach(), 'params')
-    _lib.call('diver_decoder_update', decoder.handle, p.data_ptr(), p.numel(), _s(p))
+    if first == 0:
+        _lib.call('diver_decoder_update', decoder.handle, p.data_ptr(), p.numel(), _s(p))
+    else:
+        tail = p[first:].contiguous()
+        _lib.call('diver_decoder_update_range', decoder.handle, tail.data_ptr(), first,
+                  tail.numel(), _s(p))
 
 
 def render_rays(grid, decoder, o, d, opts, with_count=True):

@@ -320,9 +320,21 @@ int diver_decoder_update(diver_decoder* d, const float* params, int64_t n_params
     REQUIRE(d && params, "null argument");
     REQUIRE(n_params == d->n_params, "decoder expects %lld parameters, got %lld",
             (long long)d->n_params, (long long)n_params);
+    return diver_decoder_update_range(d, params, 0, n_params, stream);
+}
+
+int diver_decoder_update_range(diver_decoder* d, const float* params, int64_t first,
+                               int64_t count, diver_stream_t stream) {
+    REQUIRE(d && params, "null argument");
+    REQUIRE(first >= 0 && count >= 0 && first <= d->n_params && count <= d->n_params - first,
+            "parameter range [%lld, %lld) outside the decoder's %lld parameters", (long long)first,
+            (long long)(first + count), (long long)d->n_params);
     d->packs_valid = false;
-    d->density_ch0 = false;  // w_s / b_s may have changed: density from the full integral
-    int st = hip_status(hipMemcpyAsync(d->W, params, sizeof(float) * n_params,
+    // the channel-0 density shortcut holds while w_s / b_s (the first channels + 1 parameters of a
+    // decoder without pre-layers) keep the values it was detected on
+    if (first < (int64_t)d->desc.channels + 1) d->density_ch0 = false;
+    if (count == 0) return DIVER_OK;
+    int st = hip_status(hipMemcpyAsync(d->W + first, params, sizeof(float) * count,
                                        hipMemcpyDeviceToDevice, S(stream)),
                         "decoder_update");
     if (st != DIVER_OK || !d->pack_f32) return st;

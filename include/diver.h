@@ -154,6 +154,12 @@ int diver_decoder_destroy(diver_decoder* dec);
  * host at creation). */
 int diver_decoder_update(diver_decoder* dec, const float* params, int64_t n_params,
                          diver_stream_t stream);
+/* diver_decoder_update for parameters [first, first + count) only: params points at the count new
+ * values.  A range that leaves the density row (w_s, b_s: the first channels + 1 parameters)
+ * untouched keeps the channel-0 density shortcut of a decoder created with sigma = act(F[0]), so
+ * training a colour MLP over a fixed density head keeps the cheaper density path. */
+int diver_decoder_update_range(diver_decoder* dec, const float* params, int64_t first,
+                               int64_t count, diver_stream_t stream);
 int64_t diver_decoder_param_count(const diver_decoder_desc* desc);
 
 /* Real-time decode + composite of one march round.  Replaces mlp_eval(rgba, coord, voxels, v,

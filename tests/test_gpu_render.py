@@ -155,3 +155,21 @@ def test_fp32_operands_follow_device_update(scene64, cameras, dev, hidden):
         assert float((rgb_u - rgb0).abs().max()) > 1e-3  # the update changed the image
         assert float((rgb_u - rgb_f).abs().max()) <= 1e-6
         assert float((T_u - T_f).abs().max()) <= 1e-6
+
+
+def test_range_update_keeps_channel0_density(scene64, cameras, dev):
+    """diver_decoder_update_range over the colour part only (params[C+1:]) leaves the density row
+    and its channel-0 shortcut in place: the updated decoder renders bit for bit like one created
+    from the new parameters (which detects the same shortcut)."""
+    from diver_amd import calls
+    sc = scene64
+    grid = SceneGrid.from_synthetic(sc, dev)
+    layers = S.baseline_decoder_layers(32, 64)
+    dec = Decoder(from_baseline(layers, 32))
+    new = from_baseline([(w * 0.8, b + 0.05) for w, b in layers], 32)
+    fresh = Decoder(new)
+    calls.decoder_update(dec, torch.from_numpy(np.ascontiguousarray(new.params)).to(dev), first=33)
+    for cam in (cameras[0], cameras[11]):
+        rgb_u, T_u, n_u = render_camera(grid, dec, cam, 64, 64, precision='fp32', with_count=True)
+        rgb_f, T_f, n_f = render_camera(grid, fresh, cam, 64, 64, precision='fp32', with_count=True)
+        assert torch.equal(rgb_u, rgb_f) and torch.equal(T_u, T_f) and torch.equal(n_u, n_f)

@@ -60,6 +60,15 @@ def test_adam_validation_without_gpu():
     assert st == 1 and b'betas' in L.diver_last_error()
 
 
+def test_decoder_update_range_validation_without_gpu():
+    from diver_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip('libdiver.so not built')
+    L = _lib.lib()
+    assert L.diver_decoder_update_range(None, None, 0, 0, None) == 1
+    assert b'null' in L.diver_last_error()
+
+
 def test_decoder_validation_without_gpu():
     import ctypes as C
     import numpy as np
