@@ -21,7 +21,7 @@ EXPORTS = [
     'diver_camera_rays', 'diver_block_bits', 'diver_walk_rays', 'diver_render_rows',
     'diver_decoder_update', 'diver_brick_bits', 'diver_render_batch', 'diver_alpha_map',
     'diver_corner_keys', 'diver_integrate_table', 'diver_integrate_table_backward',
-    'diver_adam_step', 'diver_decoder_update_range',
+    'diver_adam_step', 'diver_decoder_update_range', 'diver_ray_suffix',
 ]
 
 MAX_REQUESTS = 16
@@ -82,6 +82,7 @@ def _declare(L):
         'diver_decoder_destroy': [vp],
         'diver_decoder_update': [vp, vp, i64, vp],
         'diver_decoder_update_range': [vp, vp, i64, i64, vp],
+        'diver_ray_suffix': [i64, vp, vp, vp, vp, vp, vp, vp],
         'diver_mlp_eval': [vp, i64, vp, vp, vp, i32, vp, vp, vp, fl, fl, vp],
         'diver_render_rays': [ctypes.POINTER(Grid), vp, ctypes.POINTER(RenderOpts), i64, vp, vp,
                               vp, vp, vp],

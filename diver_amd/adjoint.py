@@ -106,15 +106,10 @@ class _FusedRender(Function):
         r2 = torch.relu(y2)
         c = torch.sigmoid(r2 @ Wo.T + bo)
         w = Tb * a
-        # suffix sums within each ray (float64 segmented cumsum; the scan runs along the
-        # contiguous dimension of a [3, K] tensor)
-        wc = (w[None, :] * c.T).double()
-        cs = torch.cat([torch.zeros(3, 1, dtype=torch.float64, device=dev),
-                        torch.cumsum(wc, 1)], 1).T
-        incl = cs[1:] - cs[row_off][ray]
-        tot = (cs[row_off + cnt64] - cs[row_off])[ray]
-        bg = rgba[:, 3].double()[ray] if ctx.white_back else torch.zeros_like(Tb, dtype=torch.float64)
-        suffix = (tot - incl + bg[:, None]).float()
+        # suffix sums within each ray: sum over the later rows of w c, plus the background
+        # (one lane per ray, FP64 sums)
+        suffix = calls.ray_suffix(row_off, count, w, c,
+                                  rgba[:, 3].contiguous() if ctx.white_back else None)
         gk = g.contiguous()[ray]
         t_after = Tb * (1.0 - a)
         dsigma = (gk * (t_after[:, None] * c - suffix)).sum(1)

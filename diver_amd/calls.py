@@ -228,6 +228,30 @@ def walk_rays(grid, o, d, traversal='realtime', max_n=None):
     return isect, num
 
 
+def ray_suffix(row_offset, count, weight, color, background=None):
+    """Per-ray suffix sums over the adjoint's rows (diver_ray_suffix): out[k] = sum over the rows
+    j > k of the same ray of weight[j] * color[j], plus background[ray]; [K, 3] float32."""
+    tensors = [row_offset, count, weight, color] + ([background] if background is not None else [])
+    dev = _dev(*tensors)
+    count = _i32(count, 'count')
+    if row_offset.dtype != torch.int64:
+        raise TypeError('row_offset must be int64')
+    row_offset = row_offset.contiguous()
+    w, c = _f32(weight, 'weight'), _f32(color, 'color')
+    if c.dim() != 2 or c.shape[1] != 3 or w.shape[0] != c.shape[0]:
+        raise ValueError('color must be [K, 3] with K = weight.shape[0]')
+    if count.numel() != row_offset.numel():
+        raise ValueError('count and row_offset sizes differ')
+    bg = _f32(background, 'background') if background is not None else None
+    if bg is not None and bg.numel() != count.numel():
+        raise ValueError('background must have one value per ray')
+    out = torch.empty((w.shape[0], 3), dtype=torch.float32, device=dev)
+    _lib.call('diver_ray_suffix', count.numel(), row_offset.data_ptr(), count.data_ptr(),
+              w.data_ptr(), c.data_ptr(), bg.data_ptr() if bg is not None else None,
+              out.data_ptr(), _s(w))
+    return out
+
+
 def render_rows(grid, decoder, o, d, opts, count, row_offset, n_rows):
     """Recompute pass of the adjoint: rows [n_rows, 9 + C] (see include/diver.h)."""
     dev = _dev(grid.voxels, grid.mask, o, d, count, row_offset)

@@ -69,6 +69,42 @@ __global__ void __launch_bounds__(64) rows_kernel(RenderArgs a, const int32_t* _
     }
 }
 
+// Per-ray suffix sums of the compositing adjoint: for row k of ray r (rows [off_r, off_r + cnt_r)),
+// suffix[k] = sum over the ray's rows j > k of w_j c_j, plus bg_r.  One lane per ray runs backwards
+// over its rows with FP64 sums (the rows of a ray are a few to a few hundred).
+__global__ void __launch_bounds__(256) ray_suffix_kernel(int64_t R, const int64_t* __restrict__ off,
+                                                         const int32_t* __restrict__ cnt,
+                                                         const float* __restrict__ w,
+                                                         const float* __restrict__ c,
+                                                         const float* __restrict__ bg,
+                                                         float* __restrict__ out) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    const int64_t o = off[r];
+    const int n = cnt[r];
+    const double b = bg ? (double)bg[r] : 0.0;
+    double s0 = b, s1 = b, s2 = b;
+#pragma unroll 4
+    for (int k = n - 1; k >= 0; --k) {
+        const int64_t i = o + k;
+        out[3 * i + 0] = (float)s0;
+        out[3 * i + 1] = (float)s1;
+        out[3 * i + 2] = (float)s2;
+        const double wk = (double)w[i];
+        s0 += wk * (double)c[3 * i + 0];
+        s1 += wk * (double)c[3 * i + 1];
+        s2 += wk * (double)c[3 * i + 2];
+    }
+}
+
+hipError_t launch_ray_suffix(int64_t R, const int64_t* off, const int32_t* cnt, const float* w,
+                             const float* c, const float* bg, float* out, hipStream_t s) {
+    if (R <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ray_suffix_kernel, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, s, R, off,
+                       cnt, w, c, bg, out);
+    return hipGetLastError();
+}
+
 template <class D>
 static hipError_t launch_rows_d(const RenderArgs& a, const int32_t* count, const int64_t* off,
                                 float* rows, hipStream_t s) {

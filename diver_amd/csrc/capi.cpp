@@ -626,6 +626,17 @@ int diver_render_rows(const diver_grid* grid, const diver_decoder* dec,
                       "render_rows");
 }
 
+int diver_ray_suffix(int64_t n_rays, const int64_t* row_offset, const int32_t* count,
+                     const float* weight, const float* color, const float* background,
+                     float* suffix, diver_stream_t stream) {
+    REQUIRE(n_rays >= 0, "n_rays < 0");
+    if (n_rays == 0) return DIVER_OK;
+    REQUIRE(row_offset && count && weight && color && suffix, "null pointer");
+    return hip_status(diver::launch_ray_suffix(n_rays, row_offset, count, weight, color, background,
+                                               suffix, S(stream)),
+                      "ray_suffix");
+}
+
 int diver_camera_rays(const float* c2w_host, int width, int height, float focal, float* o,
                       float* d, diver_stream_t stream) {
     REQUIRE(c2w_host && o && d, "null pointer");

@@ -276,6 +276,16 @@ int diver_render_rows(const diver_grid* grid, const diver_decoder* dec,
                       const float* d, const int32_t* count, const int64_t* row_offset,
                       float* rows, diver_stream_t stream);
 
+/* Per-ray suffix sums for the compositing adjoint over the rows of diver_render_rows: for row k of
+ * ray r (rows row_offset[r] .. row_offset[r] + count[r] - 1),
+ *   suffix[k][ch] = sum over the ray's rows j > k of weight[j] * color[j][ch]  (+ background[r])
+ * with FP64 sums; background may be NULL (no background term).  This is the part of the gradient of
+ * the front-to-back composite (model/diver.py:196-202, cumprod weights) that torch autograd derives
+ * through cumprod in the reference; here dC/dsigma_k = T_after_k c_k - suffix[k]. */
+int diver_ray_suffix(int64_t n_rays, const int64_t* row_offset, const int32_t* count,
+                     const float* weight, const float* color, const float* background,
+                     float* suffix, diver_stream_t stream);
+
 /* 8^3-block occupancy bits of an [n^3] mask (n a multiple of 8): bit b of word b/32 is set
  * when block b = (bz*(n/8) + by)*(n/8) + bx holds an occupied voxel; bits has
  * ceil((n/8)^3 / 32) words.  No counterpart in the reference (an acceleration structure for

@@ -179,3 +179,30 @@ def test_training_steps_reduce_loss(dev):
         opt.step()
         losses.append(loss.item())
     assert losses[-1] < 0.7 * losses[0]
+
+
+def test_ray_suffix_matches_segmented_sums(dev):
+    """diver_ray_suffix against an FP64 loop over ragged rays (empty rays included), with and
+    without the background term: the kernel sums in FP64 and rounds once to FP32."""
+    from diver_amd import calls
+    g = torch.Generator().manual_seed(0)
+    R = 300
+    cnt = torch.randint(0, 40, (R,), generator=g).to(torch.int32)
+    cnt[::7] = 0
+    off = torch.cumsum(cnt.long(), 0) - cnt.long()
+    K = int(cnt.sum())
+    w = torch.rand(K, generator=g)
+    c = torch.rand(K, 3, generator=g)
+    bg = torch.rand(R, generator=g)
+    ref = torch.empty(K, 3, dtype=torch.float64)
+    for r in range(R):
+        o, n = int(off[r]), int(cnt[r])
+        acc = torch.zeros(3, dtype=torch.float64)
+        for k in range(n - 1, -1, -1):
+            ref[o + k] = acc
+            acc = acc + w[o + k].double() * c[o + k].double()
+    got = calls.ray_suffix(off.to(dev), cnt.to(dev), w.to(dev), c.to(dev)).cpu().double()
+    assert float((got - ref).abs().max()) <= 1e-5
+    ray = torch.repeat_interleave(torch.arange(R), cnt.long())
+    got_bg = calls.ray_suffix(off.to(dev), cnt.to(dev), w.to(dev), c.to(dev), bg.to(dev)).cpu().double()
+    assert float((got_bg - (ref + bg.double()[ray][:, None])).abs().max()) <= 1e-5
