@@ -92,3 +92,29 @@ def test_batch_limits(scene64, cameras, dev):
     outs = calls.render_batch(grid, dec, [one] * _lib.MAX_REQUESTS, opts)
     for rgba, count in outs[1:]:
         assert torch.equal(rgba, outs[0][0]) and torch.equal(count, outs[0][1])
+
+
+@pytest.mark.parametrize('precision', ['bf16', 'fp32'])
+def test_rays_per_wave_does_not_change_results(scene64, cameras, dev, precision):
+    """Small launches give each wave fewer rays (64 at >= 131,072 rays, then 32, 16, 8 down to
+    about 2,048 waves).  The same 4,096 rays rendered in launches of 4,096, 40,000, 80,000 and
+    140,000 rays (8, 16, 32 and 64 rays per wave; the rest of each launch is other rays) must come
+    out bit for bit the same."""
+    from diver_amd.renderer import render_rays
+    grid = SceneGrid.from_synthetic(scene64, dev)
+    dec = Decoder(from_baseline(S.baseline_decoder_layers(32, 32), 32))
+    o, d = camera_rays(cameras[3], 400, 350, 0.69, dev)  # 140,000 rays
+    o, d = o.reshape(-1, 3).contiguous(), d.reshape(-1, 3).contiguous()
+    idx = torch.arange(0, o.shape[0], o.shape[0] // 4096, device=dev)[:4096]
+    rest = torch.ones(o.shape[0], dtype=torch.bool, device=dev)
+    rest[idx] = False
+    others = torch.nonzero(rest).flatten()
+    results = []
+    for total in (4096, 40000, 80000, 140000):
+        sel = torch.cat([idx, others[:total - 4096]])
+        rgb, T, cnt = render_rays(grid, dec, o[sel].contiguous(), d[sel].contiguous(), precision)
+        results.append((rgb[:4096], T[:4096], cnt[:4096]))
+    assert int(results[0][2].sum()) > 0  # the rays see the scene
+    for rgb, T, cnt in results[1:]:
+        assert torch.equal(rgb, results[0][0]) and torch.equal(T, results[0][1])
+        assert torch.equal(cnt, results[0][2])
