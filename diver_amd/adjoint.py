@@ -49,17 +49,18 @@ def split_canonical(p: torch.Tensor, C: int, H: int):
 def _wgrad(a: torch.Tensor, b: torch.Tensor, chunk: int = 4096) -> torch.Tensor:
     """a.T @ b for tall a [K, m] and b [K, n] (a weight gradient: a reduction over all K rows into
     at most 64 x 64).  One GEMM of that shape has only a handful of output tiles to spread over the
-    GPU, so the K reduction is split into chunks of `chunk` rows: one batched product of the chunks,
-    then a sum over them."""
+    GPU, so the whole chunks of `chunk` rows go through one batched product and a sum over chunks;
+    the remaining rows (fewer than `chunk`) add one plain product."""
     K = a.shape[0]
-    if K <= 2 * chunk:
+    S = K // chunk
+    if S < 2:
         return a.T @ b
-    S = (K + chunk - 1) // chunk
-    pad = S * chunk - K
-    a = torch.nn.functional.pad(a, (0, 0, 0, pad)) if pad else a
-    b = torch.nn.functional.pad(b, (0, 0, 0, pad)) if pad else b
-    at = a.reshape(S, chunk, a.shape[1]).transpose(1, 2).contiguous()
-    return torch.bmm(at, b.reshape(S, chunk, b.shape[1]).contiguous()).sum(0)
+    K0 = S * chunk
+    at = a[:K0].reshape(S, chunk, a.shape[1]).transpose(1, 2).contiguous()
+    out = torch.bmm(at, b[:K0].reshape(S, chunk, b.shape[1])).sum(0)
+    if K0 < K:
+        out = out + a[K0:].T @ b[K0:]
+    return out
 
 
 class _FusedRender(Function):
@@ -96,7 +97,7 @@ class _FusedRender(Function):
         rows = calls.render_rows(grid, decoder, o, d, ctx.opts, count, row_off, K)
         ray = torch.repeat_interleave(torch.arange(R, device=dev), cnt64)
         pin, pout = rows[:, 0:3], rows[:, 3:6]
-        s, Tb, a, F = rows[:, 6], rows[:, 7], rows[:, 8], rows[:, 9:]
+        s, Tb, a, F = rows[:, 6], rows[:, 7], rows[:, 8], rows[:, 9:].contiguous()
         ws, bs, Wc, Wv, bc, W2, b2, Wo, bo = split_canonical(params, C, H)
         pe = positional_encoding(d)[ray]
         # decoder forward on the rows
