@@ -47,6 +47,8 @@ def main(argv=None):
     ap.add_argument('--streams', type=int, default=2)
     ap.add_argument('--baseline', action='store_true',
                     help='one request per launch, one launch in flight, FP32 decoder')
+    ap.add_argument('--split-rays', type=int, default=1 << 18,
+                    help='frames above this many rays launch apart from smaller ones (0: mix)')
     args = ap.parse_args(argv)
     max_requests = 16
     if args.baseline:
@@ -56,7 +58,8 @@ def main(argv=None):
     scenes = build_scenes(dev)
     cams = S.hemisphere_cameras()
     rng = np.random.default_rng(S.SEED_ARRIVALS)
-    with RenderServer(dev, n_streams=args.streams, max_requests=max_requests) as srv:
+    with RenderServer(dev, n_streams=args.streams, max_requests=max_requests,
+                      split_rays=args.split_rays or None) as srv:
         for name, (g, d) in scenes.items():
             srv.add_scene(name, g, d, precision=args.precision)
         # unloaded latency of each resolution (scene 256^3 x 32, one frame at a time)

@@ -104,7 +104,8 @@ class _Launch:
 
 class RenderServer:
     def __init__(self, device=None, hbm_budget: Optional[int] = None, n_streams: int = 2,
-                 max_requests: int = _lib.MAX_REQUESTS, max_rays: int = 1 << 23):
+                 max_requests: int = _lib.MAX_REQUESTS, max_rays: int = 1 << 23,
+                 split_rays: Optional[int] = 1 << 18):
         self.device = torch.device(device or 'cuda:0')
         if hbm_budget is None:
             free, _total = torch.cuda.mem_get_info(self.device)
@@ -112,6 +113,9 @@ class RenderServer:
         self.hbm_budget = int(hbm_budget)
         self.max_requests = max(1, min(int(max_requests), _lib.MAX_REQUESTS))
         self.max_rays = int(max_rays)
+        # frames above split_rays rays (512x512) and frames up to it go to separate launches, so
+        # a small frame never waits for a large one's launch to finish; None mixes them
+        self.split_rays = None if split_rays is None else int(split_rays)
         self.streams = [torch.cuda.Stream(device=self.device) for _ in range(max(1, n_streams))]
         self.scenes: Dict[str, ServedScene] = {}
         self._resident = 0
@@ -175,16 +179,20 @@ class RenderServer:
         self.close()
 
     # ---- render loop ----------------------------------------------------------------------
+    def _size_class(self, req: FrameRequest) -> bool:
+        return self.split_rays is not None and req.rays > self.split_rays
+
     def _take_batch(self) -> List[FrameRequest]:
-        """Pending requests of the oldest request's scene that fit the launch and memory limits
-        (FIFO within the scene).  Called with the lock held."""
+        """Pending requests of the oldest request's scene and size class that fit the launch and
+        memory limits (FIFO within the scene).  Called with the lock held."""
         if not self._pending:
             return []
         scene = self._pending[0].scene
+        size_class = self._size_class(self._pending[0])
         free = self.hbm_budget - self._resident - self._inflight_bytes
         batch, rays, nbytes = [], 0, 0
         for req in list(self._pending):
-            if req.scene != scene:
+            if req.scene != scene or self._size_class(req) != size_class:
                 continue
             if (len(batch) == self.max_requests or rays + req.rays > self.max_rays or
                     nbytes + req.out_bytes > free):
