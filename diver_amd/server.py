@@ -2,8 +2,9 @@
 
 Scenes (a SceneGrid plus a Decoder) stay resident in device memory.  Clients submit frame
 requests (scene, camera, resolution) from any thread; a render loop thread packs the pending
-requests of one scene (FIFO, up to ``max_requests`` requests and ``max_rays`` rays) into one
-``diver_render_batch`` launch and keeps up to ``n_streams`` launches in flight.  Requests that
+requests of one scene (FIFO, up to ``max_requests`` requests and ``max_rays`` rays; frames above
+``split_rays`` rays apart from smaller ones) into one ``diver_render_batch`` launch and keeps up to
+``n_streams`` launches in flight.  Requests that
 arrive while launches run are admitted into the next launch as soon as a stream slot and memory
 are free, so their work overlaps the frames already on the GPU.  A second thread waits on the
 oldest launch's completion event (blocking, without polling) and releases its frames and its
