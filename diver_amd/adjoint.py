@@ -46,6 +46,9 @@ def split_canonical(p: torch.Tensor, C: int, H: int):
     return (ws, bs, wc.view(H, C), wv.view(H, PE_DIM), bc, w2.view(H, H), b2, wo.view(3, H), bo)
 
 
+_ROW_BUCKET = 32768
+
+
 def _wgrad(a: torch.Tensor, b: torch.Tensor, chunk: int = 4096) -> torch.Tensor:
     """a.T @ b for tall a [K, m] and b [K, n] (a weight gradient: a reduction over all K rows into
     at most 64 x 64).  One GEMM of that shape has only a handful of output tiles to spread over the
@@ -94,9 +97,16 @@ class _FusedRender(Function):
             gv0 = None if ctx.accumulate is not None else torch.zeros(ctx.vshape, device=dev)
             return gv0, torch.zeros_like(params), *([None] * 8)
         calls.decoder_update(decoder, params, first=decoder.channels + 1)
-        rows = calls.render_rows(grid, decoder, o, d, ctx.opts, count, row_off, K)
-        ray = torch.repeat_interleave(torch.arange(R, device=dev), cnt64)
-        pin, pout = rows[:, 0:3], rows[:, 3:6]
+        # The row count changes every step; rounded up to a multiple of _ROW_BUCKET, the GEMMs over
+        # the rows repeat their shapes (a new shape costs the GEMM library a fresh solution lookup,
+        # ~0.6 ms a step in all).  Padding rows are zero (T = alpha = 0, F = 0, suffix 0), so every
+        # gradient term they produce is exactly zero; only the first K rows reach the grid scatter.
+        Kp = -(-K // _ROW_BUCKET) * _ROW_BUCKET
+        rows = calls.render_rows(grid, decoder, o, d, ctx.opts, count, row_off, K, capacity=Kp)
+        ray = torch.repeat_interleave(torch.arange(R, device=dev), cnt64, output_size=K)
+        if Kp > K:
+            ray = torch.cat([ray, torch.zeros(Kp - K, dtype=ray.dtype, device=dev)])
+        pin, pout = rows[:K, 0:3], rows[:K, 3:6]
         s, Tb, a, F = rows[:, 6], rows[:, 7], rows[:, 8], rows[:, 9:].contiguous()
         ws, bs, Wc, Wv, bc, W2, b2, Wo, bo = split_canonical(params, C, H)
         pe = positional_encoding(d)[ray]
@@ -109,8 +119,10 @@ class _FusedRender(Function):
         w = Tb * a
         # suffix sums within each ray: sum over the later rows of w c, plus the background
         # (one lane per ray, FP64 sums)
-        suffix = calls.ray_suffix(row_off, count, w, c,
+        suffix = calls.ray_suffix(row_off, count, w[:K], c[:K],
                                   rgba[:, 3].contiguous() if ctx.white_back else None)
+        if Kp > K:
+            suffix = torch.nn.functional.pad(suffix, (0, 0, 0, Kp - K))
         gk = g.contiguous()[ray]
         t_after = Tb * (1.0 - a)
         dsigma = (gk * (t_after[:, None] * c - suffix)).sum(1)
@@ -137,10 +149,10 @@ class _FusedRender(Function):
             leaf = ctx.accumulate
             if leaf.grad is None:
                 leaf.grad = torch.zeros_like(leaf)
-            calls.integrate_backward(ctx.vshape, pin, pout, dF.contiguous(), leaf.grad)
+            calls.integrate_backward(ctx.vshape, pin, pout, dF[:K].contiguous(), leaf.grad)
             gv = None
         else:
-            gv = calls.integrate_backward(ctx.vshape, pin, pout, dF.contiguous())
+            gv = calls.integrate_backward(ctx.vshape, pin, pout, dF[:K].contiguous())
         return gv, gp, None, None, None, None, None, None, None, None
 
 

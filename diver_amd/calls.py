@@ -252,15 +252,19 @@ def ray_suffix(row_offset, count, weight, color, background=None):
     return out
 
 
-def render_rows(grid, decoder, o, d, opts, count, row_offset, n_rows):
-    """Recompute pass of the adjoint: rows [n_rows, 9 + C] (see include/diver.h)."""
+def render_rows(grid, decoder, o, d, opts, count, row_offset, n_rows, capacity=None):
+    """Recompute pass of the adjoint: rows [n_rows, 9 + C] (see include/diver.h).  With capacity
+    > n_rows the tensor has capacity rows and rows n_rows.. are zero."""
     dev = _dev(grid.voxels, grid.mask, o, d, count, row_offset)
     o, d = _f32(o, 'o'), _f32(d, 'd')
     count = _i32(count, 'count')
     if row_offset.dtype != torch.int64:
         raise TypeError('row_offset must be int64')
     row_offset = row_offset.contiguous()
-    rows = torch.empty((int(n_rows), 9 + grid.channels), dtype=torch.float32, device=dev)
+    cap = int(n_rows) if capacity is None else max(int(capacity), int(n_rows))
+    rows = torch.empty((cap, 9 + grid.channels), dtype=torch.float32, device=dev)
+    if cap > n_rows:
+        rows[int(n_rows):].zero_()
     g = _grid_of(grid)
     _lib.call('diver_render_rows', ctypes.byref(g), decoder.handle, ctypes.byref(opts),
               o.shape[0], o.data_ptr(), d.data_ptr(), count.data_ptr(), row_offset.data_ptr(),
