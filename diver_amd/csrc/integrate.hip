@@ -54,7 +54,7 @@ __device__ __forceinline__ CornerWeights corner_weights(float x0, float y0, floa
     return r;
 }
 
-template <int LANES>  // lanes per row = C/4
+template <int LANES>  // lanes per row (lanes_for(C))
 __global__ void __launch_bounds__(256) integrate_kernel(const float* __restrict__ voxels, int n1,
                                                         int C, int64_t K,
                                                         const float* __restrict__ ci,
@@ -203,10 +203,18 @@ hipError_t launch_corner_keys(int n1, int64_t K, const float* ci, const float* c
     return hipGetLastError();
 }
 
-#define DIVER_TABLE_LAUNCH(KERNEL, ...)                                                        \
+// Lanes per row: C/4 float4 groups rounded up to a power of two (at most 16), so the grid of
+// K*lanes threads matches the kernel's row = thread / LANES for every C % 4 == 0 (C = 12 gives 4
+// lanes, one of them idle; C = 48 gives 16 lanes, each taking 3 groups).
+static int lanes_for(int C) {
+    int lanes = 1;
+    while (lanes < C / 4 && lanes < 16) lanes *= 2;
+    return lanes;
+}
+
+#define DIVER_LANES_LAUNCH(KERNEL, ...)                                                        \
     do {                                                                                       \
-        int lanes = C / 4;                                                                     \
-        if (lanes > 16) lanes = 16;                                                            \
+        const int lanes = lanes_for(C);                                                        \
         const int64_t threads = K * lanes;                                                     \
         const dim3 grid((unsigned)((threads + 255) / 256));                                    \
         switch (lanes) {                                                                       \
@@ -221,7 +229,7 @@ hipError_t launch_corner_keys(int n1, int64_t K, const float* ci, const float* c
 hipError_t launch_integrate_table(const float* table, int C, int64_t K, const int64_t* crow,
                                   const float* ci, const float* co, float* out, hipStream_t s) {
     if (K == 0) return hipSuccess;
-    DIVER_TABLE_LAUNCH(integrate_table_kernel, table, C, K, crow, ci, co, out);
+    DIVER_LANES_LAUNCH(integrate_table_kernel, table, C, K, crow, ci, co, out);
     return hipGetLastError();
 }
 
@@ -229,44 +237,24 @@ hipError_t launch_integrate_table_backward(int C, int64_t K, const int64_t* crow
                                            const float* co, const float* gout, float* gtable,
                                            hipStream_t s) {
     if (K == 0) return hipSuccess;
-    DIVER_TABLE_LAUNCH(integrate_table_backward_kernel, C, K, crow, ci, co, gout, gtable);
+    DIVER_LANES_LAUNCH(integrate_table_backward_kernel, C, K, crow, ci, co, gout, gtable);
     return hipGetLastError();
 }
-
-#undef DIVER_TABLE_LAUNCH
 
 hipError_t launch_integrate(const float* voxels, int n1, int C, int64_t K, const float* ci,
                             const float* co, float* out, hipStream_t s) {
     if (K == 0) return hipSuccess;
-    int lanes = C / 4;
-    if (lanes > 16) lanes = 16;
-    int64_t threads = K * lanes;
-    dim3 grid((unsigned)((threads + 255) / 256));
-    switch (lanes) {
-        case 1: hipLaunchKernelGGL(integrate_kernel<1>, grid, dim3(256), 0, s, voxels, n1, C, K, ci, co, out); break;
-        case 2: hipLaunchKernelGGL(integrate_kernel<2>, grid, dim3(256), 0, s, voxels, n1, C, K, ci, co, out); break;
-        case 4: hipLaunchKernelGGL(integrate_kernel<4>, grid, dim3(256), 0, s, voxels, n1, C, K, ci, co, out); break;
-        case 8: hipLaunchKernelGGL(integrate_kernel<8>, grid, dim3(256), 0, s, voxels, n1, C, K, ci, co, out); break;
-        default: hipLaunchKernelGGL(integrate_kernel<16>, grid, dim3(256), 0, s, voxels, n1, C, K, ci, co, out); break;
-    }
+    DIVER_LANES_LAUNCH(integrate_kernel, voxels, n1, C, K, ci, co, out);
     return hipGetLastError();
 }
 
 hipError_t launch_integrate_backward(int n1, int C, int64_t K, const float* ci, const float* co,
                                      const float* gout, float* gvox, hipStream_t s) {
     if (K == 0) return hipSuccess;
-    int lanes = C / 4;
-    if (lanes > 16) lanes = 16;
-    int64_t threads = K * lanes;
-    dim3 grid((unsigned)((threads + 255) / 256));
-    switch (lanes) {
-        case 1: hipLaunchKernelGGL(integrate_backward_kernel<1>, grid, dim3(256), 0, s, n1, C, K, ci, co, gout, gvox); break;
-        case 2: hipLaunchKernelGGL(integrate_backward_kernel<2>, grid, dim3(256), 0, s, n1, C, K, ci, co, gout, gvox); break;
-        case 4: hipLaunchKernelGGL(integrate_backward_kernel<4>, grid, dim3(256), 0, s, n1, C, K, ci, co, gout, gvox); break;
-        case 8: hipLaunchKernelGGL(integrate_backward_kernel<8>, grid, dim3(256), 0, s, n1, C, K, ci, co, gout, gvox); break;
-        default: hipLaunchKernelGGL(integrate_backward_kernel<16>, grid, dim3(256), 0, s, n1, C, K, ci, co, gout, gvox); break;
-    }
+    DIVER_LANES_LAUNCH(integrate_backward_kernel, n1, C, K, ci, co, gout, gvox);
     return hipGetLastError();
 }
+
+#undef DIVER_LANES_LAUNCH
 
 }  // namespace diver

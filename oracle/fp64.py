@@ -255,29 +255,40 @@ def composite(sigma: np.ndarray, color: np.ndarray, mask: np.ndarray, white_back
 
 
 def render_front_to_back(ray_index: np.ndarray, alpha: np.ndarray, color: np.ndarray,
-                         n_rays: int, t_stop: float = 1e-4, alpha_skip: float = 0.0):
+                         n_rays: int, t_stop: float = 1e-4, alpha_skip: float = 0.0,
+                         rel: float = 1e-3, return_near: bool = False):
     """Front-to-back compositing of rows sorted by (ray, depth) with early termination.
 
     A row whose alpha is below ``alpha_skip`` is skipped entirely (transmittance unchanged,
     mlp_eval.cu:153-155); a ray stops after the row that takes its transmittance below ``t_stop``
-    (mlp_eval.cu:218-222).  Returns (rgb_accum [R,3], T [R], rows_used [R]).
+    (mlp_eval.cu:218-222).  Returns (rgb_accum [R,3], T [R], rows_used [R]), and with
+    ``return_near`` also near [R]: the ray met a row whose alpha lies within ``rel`` (relative) of
+    alpha_skip or whose transmittance after the row lies within ``rel`` of t_stop, so an FP32
+    evaluation may take the other branch there (skip or not, stop or go on).
     """
     rgb = np.zeros((n_rays, 3))
     T = np.ones(n_rays)
     used = np.zeros(n_rays, np.int64)
     done = np.zeros(n_rays, bool)
+    near = np.zeros(n_rays, bool)
     for k in range(ray_index.shape[0]):
         r = ray_index[k]
         if done[r]:
             continue
         used[r] += 1
         a = alpha[k]
+        if alpha_skip > 0 and abs(a - alpha_skip) <= rel * alpha_skip:
+            near[r] = True
         if a < alpha_skip:
             continue
         rgb[r] += T[r] * a * color[k]
         T[r] = T[r] * (1.0 - a)
+        if t_stop > 0 and abs(T[r] - t_stop) <= rel * t_stop:
+            near[r] = True
         if T[r] < t_stop:
             done[r] = True
+    if return_near:
+        return rgb, T, used, near
     return rgb, T, used
 
 
@@ -306,11 +317,13 @@ def rt_blob_from_state(sd: dict, hidden: int) -> np.ndarray:
                            np.zeros(28)])
 
 
-def mlp_eval_round(coords, voxels_dense, blob, hidden, d, rgba, finish, alpha_skip, t_stop):
+def mlp_eval_round(coords, voxels_dense, blob, hidden, d, rgba, finish, alpha_skip, t_stop,
+                   near=None, rel=1e-3):
     """One mlp_eval call (mlp_eval.cu:50-230) in float64, in place on rgba / finish (numpy).
 
     coords [8, B, 6]; voxels_dense [(N+1)^3 grid, C] already in the blob's feature space
-    (pre-multiplied for hidden=32)."""
+    (pre-multiplied for hidden=32).  ``near`` (bool [B], optional) is set for rays that meet an
+    alpha within ``rel`` of alpha_skip or a transmittance within ``rel`` of t_stop."""
     B = coords.shape[1]
     for b in range(B):
         if finish[b]:
@@ -323,11 +336,15 @@ def mlp_eval_round(coords, voxels_dense, blob, hidden, d, rgba, finish, alpha_sk
             F = integrate_weights(voxels_dense, c[None, :3], c[None, 3:])
             s, rgb = rt_blob_decoder(blob, hidden, F, np.asarray(d[b:b + 1], np.float64))
             alpha = 1.0 - math.exp(-max(s[0], 0.0))
+            if near is not None and alpha_skip > 0 and abs(alpha - alpha_skip) <= rel * alpha_skip:
+                near[b] = True
             if alpha < alpha_skip:
                 continue
             T = rgba[b, 3]
             rgba[b, :3] += T * alpha * rgb[0]
             rgba[b, 3] = T * (1.0 - alpha)
+            if near is not None and t_stop > 0 and abs(rgba[b, 3] - t_stop) <= rel * t_stop:
+                near[b] = True
             if rgba[b, 3] < t_stop:
                 finish[b] = 1
                 break

@@ -34,14 +34,17 @@ def intervals(o, d, mask, n, xyzmin, voxel_size, max_n=None, walk='offline'):
 
 def render(voxels, mask, n, xyzmin, voxel_size, decode, density_act, o, d, t_stop=1e-4,
            alpha_skip=0.0, max_n=None, walk='offline'):
-    """Returns dict(rgb [R,3] without background, T [R], used [R], num [R])."""
+    """Returns dict(rgb [R,3] without background, T [R], used [R], num [R], near [R]); near marks
+    rays that meet an alpha or transmittance within 1e-3 (relative) of alpha_skip or t_stop."""
     ray, pin, pout, num = intervals(o, d, mask, n, xyzmin, voxel_size, max_n, walk)
     R = o.shape[0]
     if ray.size == 0:
-        return dict(rgb=np.zeros((R, 3)), T=np.ones(R), used=np.zeros(R, np.int64), num=num)
+        return dict(rgb=np.zeros((R, 3)), T=np.ones(R), used=np.zeros(R, np.int64), num=num,
+                    near=np.zeros(R, bool))
     F = fp64.integrate_einsum(voxels, pin, pout)
     s_raw, rgb = decode(F, np.asarray(d, np.float64)[ray])
     sigma = np.maximum(s_raw, 0.0) if density_act == 0 else fp64.softplus(s_raw)
     alpha = -np.expm1(-sigma)
-    acc, T, used = fp64.render_front_to_back(ray, alpha, rgb, R, t_stop, alpha_skip)
-    return dict(rgb=acc, T=T, used=used, num=num)
+    acc, T, used, near = fp64.render_front_to_back(ray, alpha, rgb, R, t_stop, alpha_skip,
+                                                   return_near=True)
+    return dict(rgb=acc, T=T, used=used, num=num, near=near)

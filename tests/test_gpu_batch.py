@@ -77,7 +77,11 @@ def test_batch_view_matches_oracle(scene64, cameras, dev):
     mse = float(np.mean((rgb.astype(np.float64) - rgb_ref) ** 2))
     assert -10 * np.log10(max(mse, 1e-30)) >= 45.0
     assert np.abs(rgb - rgb_ref).max() <= 2e-2
-    assert np.mean(mixed[1][1].cpu().numpy().reshape(-1) == ref['used']) >= 0.999
+    # counts equal on every ray whose FP64 transmittance never comes within 1e-3 (relative) of
+    # the 1e-4 threshold
+    cnt = mixed[1][1].cpu().numpy().reshape(-1)
+    assert ref['near'].mean() <= 0.01
+    assert np.array_equal(cnt[~ref['near']], ref['used'][~ref['near']])
 
 
 def test_batch_limits(scene64, cameras, dev):

@@ -3,8 +3,8 @@ DIVeR model contract against the FP64 oracle.
 
 Tolerances (FP32 kernels vs FP64): integral max-abs 2e-6 on features of scale 0.1; compositing
 max-abs 2e-6; mlp_eval rgba max-abs 2e-4 with thresholds off (every ray compared), and with the
-reference thresholds on, >= 99% of rays within 2e-4 (rays whose alpha or transmittance sits at a
-threshold may branch differently in FP32 and FP64).
+reference thresholds on, every ray within 2e-4 except rays whose FP64 alpha or transmittance lies
+within 1e-3 (relative) of a threshold, where FP32 and FP64 may branch differently (at most 1%).
 """
 import numpy as np
 import pytest
@@ -114,6 +114,7 @@ def test_mlp_eval_matches_fp64(scene64, cameras, dev, hidden, thresholds):
                          torch.from_numpy(octrees).to(dev), n)
     rgba_ref = rgba.cpu().numpy().astype(np.float64)
     fin_ref = finish.cpu().numpy().astype(np.uint8)
+    near = np.zeros(B, bool)
     mg = torch.from_numpy(sc.mask).to(dev)
     feats = torch.from_numpy(rows).to(dev)
     a_skip, t_stop = thresholds
@@ -122,7 +123,7 @@ def test_mlp_eval_matches_fp64(scene64, cameras, dev, hidden, thresholds):
             break
         calls.ray_march(coords, dg, mg, finish, n)
         c_np = coords.cpu().numpy()
-        fp64.mlp_eval_round(c_np, vox64, blob, hidden, d, rgba_ref, fin_ref, a_skip, t_stop)
+        fp64.mlp_eval_round(c_np, vox64, blob, hidden, d, rgba_ref, fin_ref, a_skip, t_stop, near)
         mod.mlp_eval(rgba, coords, feats, dg, finish, alpha_skip=a_skip, t_stop=t_stop)
     got = rgba.cpu().numpy()
     err = np.abs(got - rgba_ref).max(1)
@@ -130,7 +131,10 @@ def test_mlp_eval_matches_fp64(scene64, cameras, dev, hidden, thresholds):
         assert err.max() < 2e-4
         assert np.array_equal(finish.cpu().numpy().astype(np.uint8), fin_ref)
     else:
-        assert np.mean(err < 2e-4) >= 0.99
+        # every ray within 2e-4 except those whose FP64 alpha or transmittance lies within 1e-3
+        # (relative) of the 1e-2 alpha-skip or termination threshold
+        assert near.mean() <= 0.01
+        assert err[~near].max() < 2e-4, err[~near].max()
 
 
 def test_diver_model_contract(scene64, cameras, dev):
@@ -168,3 +172,45 @@ def test_diver_model_contract(scene64, cameras, dev):
     (rgb.sum() + 1e-5 * torch.log1p(sigma ** 2 / 0.5).mean()).backward()
     assert model.voxels.grad is not None and float(model.voxels.grad.abs().sum()) > 0
     assert model.mlp1[0].weight.grad is not None
+
+
+@pytest.mark.parametrize('C', [4, 12, 20, 48, 60, 64])
+def test_integrate_channel_counts(C, dev):
+    """Every C % 4 == 0 covers all rows: forward, adjoint and the table forms agree with the FP64
+    corner-weight integral at channel counts whose C/4 is not a power of two (12, 20, 48, 60)."""
+    rng = np.random.default_rng(100 + C)
+    n1, K = 9, 4099
+    vox = rng.normal(scale=0.1, size=(n1, n1, n1, C)).astype(np.float32)
+    cell = rng.integers(0, n1 - 1, (K, 3))
+    pin = (cell + rng.uniform(0, 1, (K, 3))).astype(np.float32)
+    pout = (cell + rng.uniform(0, 1, (K, 3))).astype(np.float32)
+    ref = fp64.integrate_einsum(vox.astype(np.float64), pin, pout)
+    tv = torch.from_numpy(vox).to(dev)
+    ti, to = torch.from_numpy(pin).to(dev), torch.from_numpy(pout).to(dev)
+    got = calls.integrate(tv, ti, to).cpu().numpy()
+    assert np.abs(got - ref).max() < 2e-6
+
+    # adjoint: <g, I(v)> = <I^T g, v> in FP64
+    g = rng.normal(size=(K, C)).astype(np.float32)
+    gv = calls.integrate_backward((n1, n1, n1, C), ti, to, torch.from_numpy(g).to(dev))
+    gv = gv.cpu().numpy().astype(np.float64)
+    v = rng.normal(size=(n1, n1, n1, C))
+    lhs = np.sum(g.astype(np.float64) * fp64.integrate_einsum(v, pin, pout))
+    assert abs(lhs - np.sum(gv * v)) < 1e-4 * max(1.0, abs(lhs))
+    # the same adjoint in closed form: corner weights of every row scattered in FP64
+    w, pmin = fp64.corner_weights(pin, pout)
+    ref_g = np.zeros((n1, n1, n1, C))
+    for j in range(8):
+        cx = np.minimum(pmin[:, 0] + (j & 1), n1 - 1).astype(np.int64)
+        cy = np.minimum(pmin[:, 1] + ((j >> 1) & 1), n1 - 1).astype(np.int64)
+        cz = np.minimum(pmin[:, 2] + (j >> 2), n1 - 1).astype(np.int64)
+        np.add.at(ref_g, (cz, cy, cx), w[:, j:j + 1] * g.astype(np.float64))
+    assert np.abs(gv - ref_g).max() < 1e-4
+
+    # table forms over the dense grid as a table of n1^3 corners
+    keys = calls.corner_keys(n1, ti, to)
+    tab = torch.from_numpy(vox.reshape(-1, C)).to(dev)
+    got_t = calls.integrate_table(tab, keys, ti, to).cpu().numpy()
+    assert np.abs(got_t - ref).max() < 2e-6
+    gt = calls.integrate_table_backward(n1 ** 3, keys, ti, to, torch.from_numpy(g).to(dev))
+    np.testing.assert_allclose(gt.cpu().numpy().reshape(gv.shape), gv, rtol=1e-5, atol=1e-4)

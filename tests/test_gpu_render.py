@@ -5,8 +5,10 @@ Stated tolerances (rgb with white background, per image, vs FP64):
   FP32 decoder: PSNR >= 80 dB, max-abs <= 1e-4
   FP16 decoder: PSNR >= 60 dB, max-abs <= 5e-3
   BF16 decoder: PSNR >= 45 dB, max-abs <= 2e-2
-Interval counts (intervals composited before T < 1e-4 or exit) must be equal except for rays
-whose FP64 transmittance crosses the threshold within 1e-3 relative of it.
+Interval counts (intervals composited before T < 1e-4 or exit) must be equal on every ray
+except those whose FP64 transmittance after some interval lies within 1e-3 (relative) of the
+threshold (the oracle's ``near`` flag), where FP32 may stop one interval earlier or later; such
+rays must stay below 1% of a view.
 """
 import numpy as np
 import pytest
@@ -40,8 +42,13 @@ def _oracle(sc, dec_kind, layers, o, d, t_stop=1e-4, walk='realtime'):
                          walk=walk)
 
 
-def _count_ok(got, ref):
-    return np.mean(got == ref)
+def _check_counts(got, ref):
+    """Counts equal on every ray the FP64 oracle does not flag as threshold-adjacent."""
+    got = np.asarray(got).reshape(-1)
+    near = ref['near']
+    assert near.mean() <= 0.01, near.mean()
+    bad = np.nonzero((got != ref['used']) & ~near)[0]
+    assert bad.size == 0, (bad[:8], got[bad[:8]], ref['used'][bad[:8]])
 
 
 @pytest.mark.parametrize('walk', ['realtime', 'offline'])
@@ -64,7 +71,7 @@ def test_render_camera_baseline_decoder(scene64, cameras, dev, hidden, precision
         err = np.abs(rgb.cpu().numpy() - rgb_ref).max()
         min_psnr, max_abs = TOL[precision]
         assert p >= min_psnr and err <= max_abs, (precision, p, err)
-        assert _count_ok(cnt.cpu().numpy().reshape(-1), ref['used']) >= 0.999
+        _check_counts(cnt.cpu().numpy(), ref)
 
 
 @pytest.mark.parametrize('hidden', [32, 64])
@@ -79,7 +86,7 @@ def test_render_rays_reference_decoder_fp32(scene64, cameras, dev, hidden):
     ref = _oracle(sc, 'reference', sd, o, d, walk='offline')
     rgb_ref = ref['rgb'] + ref['T'][:, None]
     assert np.abs(rgb.cpu().numpy() - rgb_ref).max() < 1e-4
-    assert _count_ok(cnt.cpu().numpy(), ref['used']) >= 0.999
+    _check_counts(cnt.cpu().numpy(), ref)
 
 
 @pytest.mark.parametrize('walk', ['realtime', 'offline'])
@@ -109,7 +116,11 @@ def test_render_alpha_skip_reference_mode(scene64, cameras, dev):
                         lambda F, dd: fp64.mlp_chain(sd, F, dd), 0, o, d, 1e-2, alpha_skip=1e-2,
                         walk='realtime')
     err = np.abs(rgb.cpu().numpy() - (ref['rgb'] + ref['T'][:, None])).max(1)
-    assert np.mean(err < 1e-4) >= 0.99
+    # every ray within 1e-4 except those whose FP64 alpha or transmittance sits within 1e-3
+    # (relative) of the 1e-2 thresholds, where FP32 may take the other branch
+    near = ref['near']
+    assert near.mean() <= 0.01
+    assert err[~near].max() < 1e-4, err[~near].max()
 
 
 def test_render_256_subsample(cameras, dev):
@@ -128,8 +139,9 @@ def test_render_256_subsample(cameras, dev):
                                   torch.from_numpy(d).to(dev), precision)
         min_psnr, max_abs = TOL[precision]
         p = psnr(rgb.cpu().numpy(), rgb_ref)
-        assert p >= min_psnr, (precision, p)
-        assert _count_ok(cnt.cpu().numpy(), ref['used']) >= 0.999
+        err = np.abs(rgb.cpu().numpy() - rgb_ref).max()
+        assert p >= min_psnr and err <= max_abs, (precision, p, err)
+        _check_counts(cnt.cpu().numpy(), ref)
 
 
 @pytest.mark.parametrize('hidden', [32, 64])

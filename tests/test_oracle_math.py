@@ -179,3 +179,60 @@ def test_composite_forms_agree():
                                           t_stop=-1.0)
     assert np.allclose(rgb, acc + T[:, None])
     assert np.allclose(weight.sum(1) + T, 1.0)
+
+
+def test_positional_encoding_closed_form():
+    """model/mlps.py:12-18 fixes the order [x, sin(pi x), cos(pi x), sin(2 pi x), cos(2 pi x), ...]
+    with x the 3-vector.  At x = (1/8, 1/4, 1/2) every band has a textbook value, distinct enough
+    that a swapped sin/cos, a wrong band order, a missing pi or an axis-major layout fails."""
+    r = np.sqrt(2.0) / 2
+    s8, c8 = np.sqrt(2 - np.sqrt(2.0)) / 2, np.sqrt(2 + np.sqrt(2.0)) / 2
+    want = np.array([
+        1 / 8, 1 / 4, 1 / 2,   # x
+        s8, r, 1.0,            # sin(pi x)
+        c8, r, 0.0,            # cos(pi x)
+        r, 1.0, 0.0,           # sin(2 pi x)
+        r, 0.0, -1.0,          # cos(2 pi x)
+        1.0, 0.0, 0.0,         # sin(4 pi x)
+        0.0, -1.0, 1.0,        # cos(4 pi x)
+        0.0, 0.0, 0.0,         # sin(8 pi x)
+        -1.0, 1.0, 1.0,        # cos(8 pi x)
+    ])
+    got = fp64.positional_encoding(np.array([[1 / 8, 1 / 4, 1 / 2]]), 4)[0]
+    assert got.shape == (27,)
+    np.testing.assert_allclose(got, want, atol=1e-12)
+
+
+def _kernel_order_encoding(d):
+    """The real-time kernels' encoding built term by term as utils/mlp_evaluation/src/
+    mlp_eval.cu:173-193 defines it: for each axis i, e = 0 the raw value, e = 1..4
+    sin(band[e-1] x_i), e = 5..8 cos(band[e-5] x_i), with band[l] = pi 2^l (mlp_eval.cu:41-42)."""
+    d = np.asarray(d, np.float64)
+    bands = [np.pi * 2.0 ** l for l in range(4)]
+    out = np.empty((d.shape[0], 27))
+    for i in range(3):
+        x = d[:, i]
+        for e in range(9):
+            if e == 0:
+                v = x
+            elif e < 5:
+                v = np.sin(bands[e - 1] * x)
+            else:
+                v = np.cos(bands[e - 5] * x)
+            out[:, 9 * i + e] = v
+    return out
+
+
+def test_positional_encoding_kernel_order():
+    """The permutation convert.py:89-92 applies to the colour layer's PE columns maps the torch
+    order onto the kernels' order: PE(d)[:, perm] equals the term-by-term kernel encoding, on
+    axis-aligned, diagonal and random directions."""
+    rng = np.random.default_rng(5)
+    d = rng.normal(size=(64, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    axes = np.vstack([np.eye(3), -np.eye(3), np.full((1, 3), 1 / np.sqrt(3.0))])
+    d = np.vstack([axes, d])
+    np.testing.assert_allclose(fp64.positional_encoding(d, 4)[:, fp64._PE_KERNEL_ORDER],
+                               _kernel_order_encoding(d), atol=1e-12)
+    # the permutation is a bijection of the 27 columns
+    assert sorted(fp64._PE_KERNEL_ORDER.tolist()) == list(range(27))

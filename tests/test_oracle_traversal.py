@@ -225,43 +225,56 @@ def test_rt_walk_equals_aabb_plus_march_rounds(scene64, cameras):
         assert np.array_equal(got, walk[b, :num[b]])
 
 
+def _voxels_of(rows):
+    """Voxel of each (entry, exit) row by the reference's corner rule (min(int(p + 1e-4)))."""
+    if len(rows) == 0:
+        return np.zeros((0, 3), np.int64)
+    return np.minimum(np.floor(rows[:, :3] + 1e-4), np.floor(rows[:, 3:] + 1e-4)).astype(np.int64)
+
+
+def _near_voxel_edge(rows, tol=2e-4):
+    """An endpoint with two coordinates within tol of integers: the ray crosses a voxel edge there,
+    where the order of the two plane crossings is not fixed in FP32."""
+    if len(rows) == 0:
+        return False
+    p = np.concatenate([rows[:, :3], rows[:, 3:]], 0).astype(np.float64)
+    return bool(((np.abs(p - np.round(p)) < tol).sum(1) >= 2).any())
+
+
 def test_real_time_march_covers_masked_traversal(scene64, cameras):
-    """aabb_intersect + ray_march rounds emit the same occupied intervals as masked_intersect
-    from the first occupied N/4 block on (ray_march.cu / ray_voxel_intersect.cu share the DDA)."""
+    """aabb_intersect + ray_march rounds emit the same occupied voxels, in the same order, as
+    masked_intersect (ray_march.cu / ray_voxel_intersect.cu share the DDA; the descent starts at
+    the first occupied block, pinned in test_oracle_descent.py, so nothing occupied lies before
+    it).  The real-time walk restarts from the descent's re-derived start point, so its endpoints
+    drift in the last bits; a ray is set aside only when one of its endpoints lies within 2e-4 of
+    a voxel edge (two coordinates near integers), where that drift can reorder two crossings.
+    Every other ray must give the identical voxel sequence and count; endpoints agree within
+    2e-3 for 99.5% of the rays and within 2e-2 for all (near-parallel crossings are ill-conditioned
+    along the ray)."""
     from diver_amd import synthetic as S
     sc = scene64
     n = sc.n
-    o, d = S.camera_rays(cameras[0], 32, 32)
-    B = o.shape[0]
-    centre = ((o[0] - np.float32(sc.xyzmin)) / np.float32(sc.voxel_size)).astype(np.float32)
     octrees = oracle.octree_levels(sc.mask)
-    coords = np.zeros((8, B, 6), np.float32)
-    finish = np.zeros(B, np.uint8)
-    rgba = np.zeros((B, 4), np.float32)
-    oracle.aabb_intersect(coords, d, centre, finish, rgba, octrees, n)
     mask = sc.mask.astype(np.uint8)
-    rows = [[] for _ in range(B)]
-    fin = finish.copy()
-    for _ in range(64):
-        if fin.all():
-            break
-        oracle.ray_march(coords, d, mask, fin, n)
+    total = aside = 0
+    errs = []
+    for cam in range(0, 200, 9):
+        o, d = S.camera_rays(cameras[cam], 64, 64)
+        B = o.shape[0]
+        walk, num = oracle.rt_walk(o, d, sc.xyzmin, sc.voxel_size, n, octrees, mask)
+        ref, rn, _ = oracle.masked_intersect(np.repeat(o[:1], B, 0), d, np.float32(sc.xyzmin),
+                                             1.0, n, np.float32(sc.voxel_size), mask, 1.0)
         for b in range(B):
-            if fin[b]:
+            if rn[b] == 0 and num[b] == 0:
                 continue
-            for h in range(8):
-                if coords[h, b, 0] < 0:
-                    fin[b] = 1
-                    break
-                rows[b].append(coords[h, b].copy())
-    assert fin.all()
-    ref, num, _ = oracle.masked_intersect(np.repeat(o[:1], B, 0), d, np.float32(sc.xyzmin), 1.0, n,
-                                          np.float32(sc.voxel_size), mask, 1.0)
-    agree = 0
-    for b in range(B):
-        got = np.array(rows[b]).reshape(-1, 6)
-        if len(got) == num[b] and (len(got) == 0 or np.allclose(got, ref[b, :num[b]], atol=2e-4)):
-            agree += 1
-    # the real-time path starts at a block entry (re-derived point) and stops at >= N, so a few
-    # rays differ in the last bits or the last interval
-    assert agree >= 0.9 * B
+            total += 1
+            got, want = walk[b, :num[b]], ref[b, :rn[b]]
+            if _near_voxel_edge(got) or _near_voxel_edge(want):
+                aside += 1
+                continue
+            assert len(got) == len(want), (cam, b)
+            assert np.array_equal(_voxels_of(got), _voxels_of(want)), (cam, b)
+            errs.append(np.abs(got - want).max())
+    assert total > 5000 and aside <= 0.02 * total
+    errs = np.array(errs)
+    assert np.mean(errs < 2e-3) >= 0.995 and errs.max() < 2e-2
