@@ -137,41 +137,112 @@ def test_mlp_eval_matches_fp64(scene64, cameras, dev, hidden, thresholds):
         assert err[~near].max() < 2e-4, err[~near].max()
 
 
-def test_diver_model_contract(scene64, cameras, dev):
-    """DIVeR.forward / render against the FP64 restatement of model/diver.py:141-207."""
+def _fp64_diver_loss(sc, sd, o, d, fine, target, l_s):
+    """The reference's training loss (train.py:102-113: MSE + l_s mean(log(1 + sigma^2/0.5)) over
+    the [B, M] sigma) through the offline path (model/diver.py:76-207) in float64 torch autograd on
+    the CPU.  Traversal from the C oracle; the integral as its corner-weight sum (a linear map of
+    the grid, so autograd yields the exact scatter); returns (rgb, sigma, mask, loss, grads)."""
+    import oracle as O
+    n, xmin, vs = sc.n, np.float32(sc.xyzmin), np.float32(sc.voxel_size)
+    if fine:
+        isect, num, tns = O.masked_intersect(o, d, xmin, xmin + n * vs, n, vs, sc.mask, 1.0,
+                                             max_n=3 * n)
+        M = int(num.max())
+        coord, tns = isect[:, :M], tns[:, :M]
+        mask = tns >= 0
+        rows = coord[mask]
+        pin, pout = rows[:, :3], rows[:, 3:]
+    else:
+        isect, num, tns = O.ray_voxel_intersect(o, d, xmin, xmin + n * vs, n, vs, max_n=3 * n)
+        K = int(num.max())
+        coord, tns = np.maximum(isect[:, :K], 0), tns[:, :K]
+        m = tns >= 0
+        mask = m[:, :-1] & m[:, 1:]
+        pin, pout = coord[:, :-1][mask], coord[:, 1:][mask]
+    vox = torch.tensor(sc.dense_voxels(), dtype=torch.float64, requires_grad=True)
+    w, pmin = fp64.corner_weights(pin, pout)
+    n1 = n + 1
+    feat = 0
+    for j in range(8):
+        z = torch.from_numpy(np.minimum(pmin[:, 2] + (j >> 2), n1 - 1))
+        y = torch.from_numpy(np.minimum(pmin[:, 1] + ((j >> 1) & 1), n1 - 1))
+        x = torch.from_numpy(np.minimum(pmin[:, 0] + (j & 1), n1 - 1))
+        feat = feat + torch.from_numpy(w[:, j:j + 1]) * vox[z, y, x]
+    P = {k: torch.tensor(v, dtype=torch.float64, requires_grad=True) for k, v in sd.items()}
+    lin = torch.nn.functional.linear
+    h = feat
+    h = torch.relu(lin(h, P['mlp1.0.weight'], P['mlp1.0.bias']))
+    h = torch.relu(lin(h, P['mlp1.2.weight'], P['mlp1.2.bias']))
+    h = lin(h, P['mlp1.4.weight'], P['mlp1.4.bias'])
+    sig_, h = h[:, 0], h[:, 1:]
+    ray = np.nonzero(mask)[0]
+    dd = torch.from_numpy(d.astype(np.float64)[ray])
+    pe = [dd]
+    for l in range(4):
+        pe += [torch.sin((2 ** l * np.pi) * dd), torch.cos((2 ** l * np.pi) * dd)]
+    h = torch.cat([h] + pe, -1)
+    h = torch.relu(lin(h, P['mlp2.0.weight'], P['mlp2.0.bias']))
+    col_ = torch.sigmoid(lin(h, P['mlp2.2.weight'], P['mlp2.2.bias']))
+    B, M = mask.shape
+    tm = torch.from_numpy(mask)
+    sigma = torch.zeros(B, M, dtype=torch.float64).masked_scatter(tm, sig_)
+    color = torch.zeros(B, M, 3, dtype=torch.float64)
+    color[tm] = col_
+    alpha = 1 - torch.exp(-torch.relu(sigma) * tm)
+    shifted = torch.cat([torch.ones(B, 1, dtype=torch.float64), 1 - alpha[:, :-1]], 1)
+    weight = alpha * torch.cumprod(shifted, -1)
+    rgb = (weight[..., None] * color).sum(1) + (1 - weight.sum(1, keepdim=True))
+    loss = torch.mean((rgb - torch.from_numpy(target)) ** 2) \
+        + l_s * torch.log(1 + sigma ** 2 / 0.5).mean()
+    loss.backward()
+    grads = {'voxels': vox.grad.numpy()}
+    grads.update({k: v.grad.numpy() for k, v in P.items()})
+    return rgb.detach().numpy(), sigma.detach().numpy(), mask, loss.item(), grads
+
+
+@pytest.mark.parametrize('fine', [1, 0])
+def test_diver_model_contract(scene64, cameras, dev, fine):
+    """DIVeR.extract_features / forward / render, masked (fine=1: masked_intersect) and unmasked
+    (fine=0: ray_voxel_intersect pairs, model/diver.py:95-103) branches, against the float64
+    restatement of model/diver.py:76-207, and the backward of the reference's training loss
+    (MSE + 1e-5 sparsity, train.py:102-113) against float64 autograd: values within 1e-4,
+    gradients within 1e-4 of the largest magnitude of each parameter's gradient."""
     from types import SimpleNamespace
     from diver_amd.model.diver import DIVeR
     hp = SimpleNamespace(voxel_num=64, voxel_dim=32, grid_size=2.0, mask_scale=1.0,
-                         white_back=True, fine=1, implicit=False, mlp_point=[32, 2, 33],
+                         white_back=True, fine=fine, implicit=False, mlp_point=[32, 2, 33],
                          mlp_view=[32, 1], dir_encode=4)
     model = DIVeR(hp).to(dev)
     sd = S.reference_decoder_state(32, 32)
     with torch.no_grad():
         model.voxels.copy_(torch.from_numpy(scene64.dense_voxels()))
-        model.voxel_mask.copy_(torch.from_numpy(scene64.mask))
+        if fine:
+            model.voxel_mask.copy_(torch.from_numpy(scene64.mask))
         for k, v in sd.items():
             dict(model.named_parameters())[k].copy_(torch.from_numpy(v))
     o, d = S.camera_rays(cameras[6], 32, 32)
-    color, sigma, mask, ts = model(torch.from_numpy(o).to(dev), torch.from_numpy(d).to(dev))
+    rng = np.random.default_rng(8)
+    target = rng.uniform(size=(o.shape[0], 3))
+    og, dg = torch.from_numpy(o).to(dev), torch.from_numpy(d).to(dev)
+    mask, feat, ts = model.extract_features(og, dg)
+    color, sigma, mask, ts = model(og, dg)
     rgb, weight = model.render(color, sigma, mask)
-    # FP64 reference
-    ray, pin, pout, num = _intervals(scene64, o, d)
-    F = fp64.integrate_einsum(scene64.dense_voxels(), pin, pout)
-    s_, c_ = fp64.mlp_chain(sd, F, d[ray].astype(np.float64))
-    B, M = mask.shape
-    assert M == num.max()
-    m_ref = np.arange(M)[None, :] < num[:, None]
+    loss = torch.mean((rgb - torch.from_numpy(target).float().to(dev)) ** 2) \
+        + 1e-5 * torch.log(1 + sigma ** 2 / 0.5).mean()
+    loss.backward()
+    rgb_ref, sig_ref, m_ref, loss_ref, g_ref = _fp64_diver_loss(scene64, sd, o, d, fine, target,
+                                                                1e-5)
     assert np.array_equal(mask.cpu().numpy(), m_ref)
-    sig = np.zeros((B, M)); sig[m_ref] = s_
-    col = np.zeros((B, M, 3)); col[m_ref] = c_
-    rgb_ref, w_ref = fp64.composite(sig, col, m_ref, True)
-    assert np.abs(sigma.detach().cpu().numpy() - sig).max() < 1e-4
+    assert feat.shape[0] == int(m_ref.sum())
+    assert np.abs(sigma.detach().cpu().numpy() - sig_ref).max() < 1e-4
     assert np.abs(rgb.detach().cpu().numpy() - rgb_ref).max() < 1e-4
-    assert np.abs(weight.detach().cpu().numpy() - w_ref).max() < 1e-4
-    # gradients reach the grid and the decoder
-    (rgb.sum() + 1e-5 * torch.log1p(sigma ** 2 / 0.5).mean()).backward()
-    assert model.voxels.grad is not None and float(model.voxels.grad.abs().sum()) > 0
-    assert model.mlp1[0].weight.grad is not None
+    assert abs(loss.item() - loss_ref) < 1e-6 * max(1.0, abs(loss_ref))
+    named = dict(model.named_parameters())
+    for k, g64 in g_ref.items():
+        g = named[k].grad.cpu().numpy().astype(np.float64)
+        scale = np.abs(g64).max()
+        assert scale > 0, k
+        assert np.abs(g - g64).max() <= 1e-4 * scale, (k, np.abs(g - g64).max(), scale)
 
 
 @pytest.mark.parametrize('C', [4, 12, 20, 48, 60, 64])

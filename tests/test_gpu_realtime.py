@@ -1,0 +1,116 @@
+"""The packaged real-time frame loop (diver_amd.realtime.RealTimeRenderer: eval_rt.py:106-122 on
+the drop-in aabb_intersect / ray_march / mlp_eval ops, the plain FP32 GPU path) and on-device
+camera rays (diver_camera_rays, utils/dataset/ray_utils.py:5-56 conventions) against references
+computed on the CPU.
+
+Stated tolerances:
+  camera rays: origins equal the camera centre exactly; directions within 4e-7 (a few ulp of a
+    unit vector) of the FP64 evaluation of ray_utils.py's formula (pixel centres at +0.5,
+    x = (i - W/2)/focal, y = -(j - H/2)/focal, z = -1, rotated by c2w[:, :3], normalised);
+  real-time loop, rgb with white background vs the FP64 oracle of the same walk: max-abs 2e-4 on
+    every ray except those whose FP64 alpha or transmittance lies within 1e-3 (relative) of the
+    alpha-skip or termination threshold (at most 1% of a view).
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import fp64
+from oracle.render import render as oracle_render
+from diver_amd import calls
+from diver_amd import synthetic as S
+from diver_amd.decoder import Decoder, from_baseline, from_state_dict
+from diver_amd.realtime import RealTimeRenderer
+
+pytestmark = pytest.mark.gpu
+
+
+def _ray_utils_fp64(c2w, W, H, focal):
+    """utils/dataset/ray_utils.py:5-56 evaluated in float64 (torch.linspace(0.5, W - 0.5, W) is
+    0.5 + k exactly)."""
+    i = (np.arange(W, dtype=np.float64) + 0.5)[None, :].repeat(H, 0)
+    j = (np.arange(H, dtype=np.float64) + 0.5)[:, None].repeat(W, 1)
+    dirs = np.stack([(i - W / 2) / focal, -(j - H / 2) / focal, -np.ones_like(i)], -1)
+    d = dirs @ np.asarray(c2w, np.float64)[:, :3].T
+    d /= np.linalg.norm(d, axis=-1, keepdims=True)
+    return d.reshape(-1, 3)
+
+
+@pytest.mark.parametrize('W,H', [(800, 800), (1920, 1080), (63, 47)])
+def test_camera_rays_follow_ray_utils(cameras, dev, W, H):
+    focal = S.focal_for(W, 0.69)
+    for cam in (cameras[0], cameras[57], cameras[199]):
+        c2w = np.asarray(cam, np.float32)
+        o, d = calls.camera_rays(c2w, W, H, focal, dev)
+        o, d = o.cpu().numpy(), d.cpu().numpy()
+        assert np.array_equal(o, np.broadcast_to(c2w[:, 3], o.shape))
+        want = _ray_utils_fp64(c2w, W, H, focal)
+        assert np.abs(d - want).max() < 4e-7
+        assert np.abs(np.linalg.norm(d.astype(np.float64), axis=1) - 1).max() < 3e-7
+
+
+def test_camera_rays_closed_forms(dev):
+    """An odd-sized image's centre pixel looks exactly down the camera's -z axis, the first pixel
+    of the first row sits at (-(W-1)/2, (H-1)/2, -focal) in camera space (top-left, y up), and the
+    half-angle of the field of view is atan(W/2 / focal) = fov/2."""
+    W, H = 801, 601
+    fov = 0.69
+    focal = S.focal_for(W, fov)
+    c2w = S.look_at_c2w(np.array([1.0, -2.0, 3.0])).astype(np.float32)
+    o, d = calls.camera_rays(c2w, W, H, focal, dev)
+    d = d.cpu().numpy().astype(np.float64).reshape(H, W, 3)
+    R = c2w[:, :3].astype(np.float64)
+    cam = d @ R  # back to camera space (R is orthonormal)
+    np.testing.assert_allclose(cam[H // 2, W // 2], [0.0, 0.0, -1.0], atol=2e-7)
+    v = np.array([-(W - 1) / 2, (H - 1) / 2, -focal])
+    np.testing.assert_allclose(cam[0, 0], v / np.linalg.norm(v), atol=3e-7)
+    assert abs(math.atan((W / 2) / focal) - fov / 2) < 1e-12
+
+
+def _check_frame(rgb, ref, near_limit=0.01, tol=2e-4):
+    rgb_ref = ref['rgb'] + ref['T'][:, None]
+    err = np.abs(np.asarray(rgb, np.float64).reshape(-1, 3) - rgb_ref).max(1)
+    near = ref['near']
+    assert near.mean() <= near_limit, near.mean()
+    assert err[~near].max() < tol, err[~near].max()
+
+
+def test_realtime_renderer_config1_reference_mode(scene64, cameras, dev):
+    """Config 1 scene (64^3 x 32), the reference DIVeR32 decoder (relu density) and the
+    reference's real-time thresholds (alpha skip 1e-2, stop at T < 1e-2), one 64x64 view."""
+    sc = scene64
+    sd = S.reference_decoder_state(32, 32)
+    dec = Decoder(from_state_dict(sd, 32))
+    rr = RealTimeRenderer(torch.from_numpy(sc.dense_voxels()).to(dev),
+                          torch.from_numpy(sc.mask).to(dev), sc.xyzmin, sc.voxel_size, dec,
+                          alpha_skip=1e-2, t_stop=1e-2)
+    for cam in (cameras[0], cameras[13]):
+        rgb, rounds = rr.render(cam, 64, 64)
+        assert rounds >= 1
+        o, d = calls.camera_rays(np.asarray(cam, np.float32), 64, 64, S.focal_for(64), dev)
+        ref = oracle_render(sc.dense_voxels(), sc.mask, sc.n, sc.xyzmin, sc.voxel_size,
+                            lambda F, dd: fp64.mlp_chain(sd, F, dd), 0, o.cpu().numpy(),
+                            d.cpu().numpy(), 1e-2, alpha_skip=1e-2, walk='realtime')
+        _check_frame(rgb.cpu().numpy(), ref)
+
+
+def test_realtime_renderer_256_strided(cameras, dev):
+    """The benchmark's plain FP32 path: 256^3 x 32 sphere/shell scene, benchmark decoder, no alpha
+    skip, stop at T < 1e-4; every 41st pixel of an 800x800 view against the FP64 oracle."""
+    sc = S.config2_scene()
+    layers = S.baseline_decoder_layers(32, 32)
+    dec = Decoder(from_baseline(layers, 32))
+    rr = RealTimeRenderer(sc.dense_voxels_torch(dev), torch.from_numpy(sc.mask).to(dev),
+                          sc.xyzmin, sc.voxel_size, dec, alpha_skip=0.0, t_stop=1e-4)
+    cam = cameras[21]
+    rgb, rounds = rr.render(cam, 800, 800)
+    o, d = calls.camera_rays(np.asarray(cam, np.float32), 800, 800, S.focal_for(800), dev)
+    sel = np.arange(0, 800 * 800, 41)
+    o, d = o.cpu().numpy()[sel], d.cpu().numpy()[sel]
+    ref = oracle_render(sc.dense_voxels(), sc.mask, sc.n, sc.xyzmin, sc.voxel_size,
+                        lambda F, dd: fp64.baseline_decoder(layers, F, dd), 1, o, d, 1e-4,
+                        walk='realtime')
+    assert int(ref['used'].sum()) > 1000
+    _check_frame(rgb.cpu().numpy().reshape(-1, 3)[sel], ref)
