@@ -22,6 +22,7 @@ EXPORTS = [
     'diver_decoder_update', 'diver_brick_bits', 'diver_render_batch', 'diver_alpha_map',
     'diver_corner_keys', 'diver_integrate_table', 'diver_integrate_table_backward',
     'diver_adam_step', 'diver_decoder_update_range', 'diver_ray_suffix',
+    'diver_render_rays_sparsity',
 ]
 
 MAX_REQUESTS = 16
@@ -86,6 +87,8 @@ def _declare(L):
         'diver_mlp_eval': [vp, i64, vp, vp, vp, i32, vp, vp, vp, fl, fl, vp],
         'diver_render_rays': [ctypes.POINTER(Grid), vp, ctypes.POINTER(RenderOpts), i64, vp, vp,
                               vp, vp, vp],
+        'diver_render_rays_sparsity': [ctypes.POINTER(Grid), vp, ctypes.POINTER(RenderOpts), i64,
+                                       vp, vp, vp, vp, vp, vp],
         'diver_render_camera': [ctypes.POINTER(Grid), vp, ctypes.POINTER(RenderOpts),
                                 ctypes.POINTER(ctypes.c_float), i32, i32, fl, vp, vp, vp],
         'diver_camera_rays': [ctypes.POINTER(ctypes.c_float), i32, i32, fl, vp, vp, vp],

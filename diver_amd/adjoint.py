@@ -3,6 +3,11 @@
 Forward: one fused kernel per batch of rays (FP32), keeping only (rgb, T) and the interval
 count per ray -- no samples are stored for the backward.
 
+With ``sparsity=True`` the forward also returns the reference's sparsity term
+mean(log(1 + sigma^2 / 0.5)) over the batch's [B, M] raw densities (M = the largest interval count
+of the batch, train.py:84-85,110; the zero padding adds nothing): the kernel sums
+log(1 + 2 s^2) per ray, and its adjoint adds g / (B M) * 4 s / (1 + 2 s^2) to each interval's ds.
+
 Backward (for decoders without pre-layers, e.g. the benchmark decoder):
   1. re-walk every ray and recompute each composited interval's integral, density and
      transmittance (``diver_render_rows``);
@@ -46,7 +51,11 @@ def split_canonical(p: torch.Tensor, C: int, H: int):
     return (ws, bs, wc.view(H, C), wv.view(H, PE_DIM), bc, w2.view(H, H), b2, wo.view(3, H), bo)
 
 
-_ROW_BUCKET = 32768
+def _row_bucket(K: int) -> int:
+    """Row capacity for K rows: the next multiple of max(4096, 2^floor(log2 K) / 8).  Shapes repeat
+    across steps (at most 8 sizes per octave) with at most 12.5 % padding above 32768 rows."""
+    step = max(4096, (1 << (max(K, 1).bit_length() - 1)) // 8)
+    return -(-K // step) * step
 
 
 def _wgrad(a: torch.Tensor, b: torch.Tensor, chunk: int = 4096) -> torch.Tensor:
@@ -69,23 +78,35 @@ def _wgrad(a: torch.Tensor, b: torch.Tensor, chunk: int = 4096) -> torch.Tensor:
 class _FusedRender(Function):
     @staticmethod
     def forward(ctx, voxels, params, grid, decoder, o, d, t_stop, white_back, traversal,
-                accumulate):
+                accumulate, sparsity):
         # the density row (w_s = e_0, b_s = 0: softplus(F[0])) is constant, so only the colour
         # part is pushed and the kernel keeps its channel-0 density path
         calls.decoder_update(decoder, params, first=decoder.channels + 1)
         opts = calls.render_opts('fp32', traversal, t_stop, 0.0)
-        rgba, count = calls.render_rays(grid, decoder, o, d, opts)
-        ctx.save_for_backward(params, o, d, rgba, count)
+        if sparsity:
+            rgba, count, sreg = calls.render_rays_sparsity(grid, decoder, o, d, opts)
+            # B x M with M the batch's largest interval count, kept on the device (no sync)
+            bm = (count.max().to(torch.float32) * o.shape[0]).clamp_min(1.0)
+        else:
+            rgba, count = calls.render_rays(grid, decoder, o, d, opts)
+            bm = torch.ones((), device=o.device)
+        ctx.save_for_backward(params, o, d, rgba, count, bm)
         ctx.grid, ctx.decoder, ctx.opts = grid, decoder, opts
         ctx.white_back = white_back
         ctx.vshape = tuple(voxels.shape)
         ctx.accumulate = voxels if accumulate else None
+        ctx.sparsity = sparsity
         bg = rgba[:, 3:] if white_back else torch.zeros_like(rgba[:, 3:])
-        return rgba[:, :3] + bg
+        rgb = rgba[:, :3] + bg
+        if sparsity:
+            return rgb, sreg.to(torch.float64).sum().to(torch.float32) / bm
+        return rgb
 
     @staticmethod
-    def backward(ctx, g):
-        params, o, d, rgba, count = ctx.saved_tensors
+    def backward(ctx, g, g_reg=None):
+        params, o, d, rgba, count, bm = ctx.saved_tensors
+        if g is None:
+            g = torch.zeros_like(rgba[:, :3])
         grid, decoder = ctx.grid, ctx.decoder
         C, H = decoder.channels, decoder.hidden
         R = o.shape[0]
@@ -95,13 +116,14 @@ class _FusedRender(Function):
         dev = o.device
         if K == 0:
             gv0 = None if ctx.accumulate is not None else torch.zeros(ctx.vshape, device=dev)
-            return gv0, torch.zeros_like(params), *([None] * 8)
+            return gv0, torch.zeros_like(params), *([None] * 9)
         calls.decoder_update(decoder, params, first=decoder.channels + 1)
-        # The row count changes every step; rounded up to a multiple of _ROW_BUCKET, the GEMMs over
-        # the rows repeat their shapes (a new shape costs the GEMM library a fresh solution lookup,
-        # ~0.6 ms a step in all).  Padding rows are zero (T = alpha = 0, F = 0, suffix 0), so every
-        # gradient term they produce is exactly zero; only the first K rows reach the grid scatter.
-        Kp = -(-K // _ROW_BUCKET) * _ROW_BUCKET
+        # The row count changes every step; rounded up to a bucket size (_row_bucket), the GEMMs
+        # over the rows repeat their shapes (a new shape costs the GEMM library a fresh solution
+        # lookup, ~0.6 ms a step in all).  Padding rows are zero (T = alpha = 0, F = 0, s = 0,
+        # suffix 0), so every gradient term they produce is exactly zero; only the first K rows
+        # reach the grid scatter.
+        Kp = _row_bucket(K)
         rows = calls.render_rows(grid, decoder, o, d, ctx.opts, count, row_off, K, capacity=Kp)
         ray = torch.repeat_interleave(torch.arange(R, device=dev), cnt64, output_size=K)
         if Kp > K:
@@ -128,6 +150,9 @@ class _FusedRender(Function):
         dsigma = (gk * (t_after[:, None] * c - suffix)).sum(1)
         act = decoder.canon.density_act
         ds = dsigma * (torch.sigmoid(s) if act == 1 else (s > 0).to(s.dtype))
+        if ctx.sparsity and g_reg is not None:
+            # d/ds log(1 + s^2 / 0.5) = 4 s / (1 + 2 s^2); padding rows have s = 0
+            ds = ds + (g_reg / bm) * (4.0 * s / (1.0 + 2.0 * s * s))
         dc = gk * w[:, None]
         do = dc * c * (1.0 - c)
         dWo = _wgrad(do, r2)
@@ -153,7 +178,7 @@ class _FusedRender(Function):
             gv = None
         else:
             gv = calls.integrate_backward(ctx.vshape, pin, pout, dF[:K].contiguous())
-        return gv, gp, None, None, None, None, None, None, None, None
+        return gv, gp, None, None, None, None, None, None, None, None, None
 
 
 class BenchmarkDecoder(nn.Module):
@@ -202,8 +227,12 @@ class FusedRenderer(nn.Module):
         self.traversal = traversal
         self.accumulate_grad = accumulate_grad
 
-    def forward(self, o, d, t_stop: float = 1e-4, white_back: bool = True):
+    def forward(self, o, d, t_stop: float = 1e-4, white_back: bool = True,
+                sparsity: bool = False):
+        """rgb [R, 3]; with ``sparsity`` also the reference's sigma_loss (train.py:84-85), the
+        mean of log(1 + sigma^2 / 0.5) over the batch's [R, max count] raw densities, so a
+        training step is ``mse(rgb, target) + l_s * sigma_loss`` as in train.py:109-113."""
         self.grid.voxels = self.voxels.data
         return _FusedRender.apply(self.voxels, self.decoder.canonical(), self.grid, self.handle,
                                   o.contiguous(), d.contiguous(), t_stop, white_back,
-                                  self.traversal, self.accumulate_grad)
+                                  self.traversal, self.accumulate_grad, bool(sparsity))

@@ -299,6 +299,22 @@ def render_rays(grid, decoder, o, d, opts, with_count=True):
     return rgba, count
 
 
+def render_rays_sparsity(grid, decoder, o, d, opts):
+    """render_rays plus, per ray, the sum of log(1 + s^2 / 0.5) over its evaluated intervals
+    (s = density pre-activation): (rgba [R, 4], count [R], sparsity [R])."""
+    dev = _dev(grid.voxels, grid.mask, o, d)
+    o, d = _f32(o, 'o'), _f32(d, 'd')
+    R = o.shape[0]
+    rgba = torch.empty((R, 4), dtype=torch.float32, device=dev)
+    count = torch.empty(R, dtype=torch.int32, device=dev)
+    sparsity = torch.empty(R, dtype=torch.float32, device=dev)
+    g = _grid_of(grid)
+    _lib.call('diver_render_rays_sparsity', ctypes.byref(g), decoder.handle, ctypes.byref(opts),
+              R, o.data_ptr(), d.data_ptr(), rgba.data_ptr(), count.data_ptr(),
+              sparsity.data_ptr(), _s(o))
+    return rgba, count, sparsity
+
+
 def corner_keys(n1, coord_in, coord_out):
     """int64 [K, 8]: keys z n1^2 + y n1 + x of the 8 corners of each row's integral."""
     dev = _dev(coord_in, coord_out)

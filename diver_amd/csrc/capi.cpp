@@ -522,6 +522,42 @@ int diver_render_rays(const diver_grid* grid, const diver_decoder* dec,
     return dispatch_render(dec, a, opts->precision, S(stream), "render_rays");
 }
 
+int diver_render_rays_sparsity(const diver_grid* grid, const diver_decoder* dec,
+                               const diver_render_opts* opts, int64_t n_rays, const float* o,
+                               const float* d, float* rgba, int32_t* count, float* sparsity,
+                               diver_stream_t stream) {
+    diver::RenderArgs a{};
+    int st = render_common(grid, dec, opts, a);
+    if (st != DIVER_OK) return st;
+    REQUIRE(n_rays >= 0, "n_rays < 0");
+    REQUIRE(n_rays == 0 || (o && d && rgba && sparsity), "null pointer");
+    a.n_rays = n_rays;
+    a.o = o;
+    a.d = d;
+    a.cam.width = 0;
+    a.rgba = rgba;
+    a.count = count;
+    if (!tiles_ok(dec, a))
+        return fail(DIVER_EUNSUPPORTED,
+                    "the sparsity output needs the tile kernel (block and brick bits, a decoder "
+                    "without pre-layers)");
+    diver::BatchArgs p{};
+    p.g = a;
+    p.n_req = 1;
+    p.req[0].cam = a.cam;
+    p.req[0].n_rays = n_rays;
+    p.req[0].o = o;
+    p.req[0].d = d;
+    p.req[0].rgba = rgba;
+    p.req[0].count = count;
+    p.req[0].sparsity = sparsity;
+    p.total = n_rays;
+    p.sparsity = 1;
+    return hip_status(diver::launch_render_tiles(dec->variant, p, opts->precision,
+                                                 dec->density_ch0, S(stream)),
+                      "render_rays_sparsity");
+}
+
 int diver_alpha_map(const diver_grid* grid, const diver_decoder* dec, int64_t n_rays,
                     const float* o, const float* d, int pad, float* alpha_map,
                     diver_stream_t stream) {

@@ -45,6 +45,7 @@ struct ReqDesc {
     const float* d;
     float* rgba;
     int32_t* count;
+    float* sparsity;  // [rays] sum of log(1 + 2 s^2) over the evaluated intervals (optional)
 };
 
 constexpr int kMaxRequests = 16;
@@ -57,6 +58,7 @@ struct BatchArgs {
     int64_t total;                 // ray ids over all requests
     int rays_per_wave;             // ids per wavefront (64, or fewer for small launches; set by
                                    // the launcher)
+    int sparsity;                  // some request has a sparsity output
 };
 
 // Rays per wavefront for a launch of `total` rays: 64 when the launch fills the GPU, else halved

@@ -362,6 +362,7 @@ __global__ void __launch_bounds__(64 * kWavesPerGroup, tile_waves_per_simd<D>())
     Kahan S;
     float T = 1.0f;
     int used = 0;
+    float reg = 0.0f;  // sparsity term: sum of log(1 + s^2 / 0.5) over the evaluated intervals
     int qn = 0;  // queued rows (wave-uniform)
     const float nf = (float)n;
 
@@ -407,6 +408,7 @@ __global__ void __launch_bounds__(64 * kWavesPerGroup, tile_waves_per_simd<D>())
                             float sigma, alpha;
                             density_alpha<D>(s, sigma, alpha);
                             used += 1;
+                            if (p.sparsity) reg += log1pf(2.0f * s * s);
                             if (alpha >= a.alpha_skip) {
                                 wgt = T * alpha;
                                 push = true;
@@ -495,6 +497,7 @@ __global__ void __launch_bounds__(64 * kWavesPerGroup, tile_waves_per_simd<D>())
         reinterpret_cast<float4*>(rq.rgba)[out_idx] =
             make_float4(acc[lane * 3], acc[lane * 3 + 1], acc[lane * 3 + 2], T);
         if (rq.count) rq.count[out_idx] = used;
+        if (rq.sparsity) rq.sparsity[out_idx] = reg;
     }
 }
 

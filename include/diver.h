@@ -215,6 +215,17 @@ int diver_render_rays(const diver_grid* grid, const diver_decoder* dec,
                       const diver_render_opts* opts, int64_t n_rays, const float* o,
                       const float* d, float* rgba, int32_t* count, diver_stream_t stream);
 
+/* diver_render_rays plus, per ray, sparsity[r] = sum over the intervals the ray evaluates of
+ * log(1 + s^2 / 0.5), s the density pre-activation of the interval: the per-ray part of the
+ * reference's sparsity regulariser l_s * mean(log(1 + sigma^2 / 0.5)) over the [B, M] sigma of a
+ * training batch (train.py:84-85,110; its zero padding adds nothing).  With t_stop = 0 and
+ * alpha_skip = 0 the intervals are all of the ray's occupied intervals, as in the reference's
+ * training forward.  Needs the tile path (block and brick bits, a decoder without pre-layers). */
+int diver_render_rays_sparsity(const diver_grid* grid, const diver_decoder* dec,
+                               const diver_render_opts* opts, int64_t n_rays, const float* o,
+                               const float* d, float* rgba, int32_t* count, float* sparsity,
+                               diver_stream_t stream);
+
 /* Occupancy extraction (replaces the alpha-map loop of prune.py:83-143: masked_intersect or
  * ray_voxel_intersect, DIVeR.decode, DIVeR.render's weights and torch_scatter's max).  Every
  * occupied voxel a ray crosses (the offline walk over grid->mask; pass an all-ones mask for the
