@@ -22,8 +22,15 @@ EXPORTS = [
     'diver_decoder_update', 'diver_brick_bits', 'diver_render_batch', 'diver_alpha_map',
     'diver_corner_keys', 'diver_integrate_table', 'diver_integrate_table_backward',
     'diver_adam_step', 'diver_decoder_update_range', 'diver_ray_suffix',
-    'diver_render_rays_sparsity',
+    'diver_render_rays_sparsity', 'diver_server_clock', 'diver_server_create',
+    'diver_server_destroy', 'diver_server_add_scene', 'diver_server_submit', 'diver_server_wait',
+    'diver_server_take_completed', 'diver_server_history',
 ]
+
+# status codes of include/diver.h
+DIVER_EBUDGET = 5
+DIVER_ETIMEDOUT = 6
+DIVER_ESTATE = 7
 
 MAX_REQUESTS = 16
 
@@ -55,6 +62,19 @@ class Grid(ctypes.Structure):
 class RenderOpts(ctypes.Structure):
     _fields_ = [('precision', ctypes.c_int), ('traversal', ctypes.c_int),
                 ('t_stop', ctypes.c_float), ('alpha_skip', ctypes.c_float)]
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [('hbm_budget', ctypes.c_int64), ('n_streams', ctypes.c_int),
+                ('max_requests', ctypes.c_int), ('max_rays', ctypes.c_int64),
+                ('split_rays', ctypes.c_int64)]
+
+
+class FrameInfo(ctypes.Structure):
+    _fields_ = [('ticket', ctypes.c_int64), ('scene', ctypes.c_int), ('width', ctypes.c_int),
+                ('height', ctypes.c_int), ('status', ctypes.c_int), ('launch', ctypes.c_int64),
+                ('t_submit', ctypes.c_double), ('t_launch', ctypes.c_double),
+                ('t_done', ctypes.c_double)]
 
 
 class Request(ctypes.Structure):
@@ -106,10 +126,26 @@ def _declare(L):
         'diver_adam_step': [vp, vp, vp, vp, i64, ctypes.c_double, ctypes.c_double, fl, fl, fl, fl,
                             i32, vp],
     }
+    sig.update({
+        'diver_server_create': [ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)],
+        'diver_server_destroy': [vp],
+        'diver_server_add_scene': [vp, ctypes.POINTER(Grid), vp, ctypes.POINTER(RenderOpts), i64,
+                                   vp, ctypes.POINTER(ctypes.c_int)],
+        'diver_server_submit': [vp, i32, ctypes.POINTER(ctypes.c_float), i32, i32, fl, vp, vp, vp,
+                                ctypes.POINTER(ctypes.c_int64)],
+        'diver_server_wait': [vp, i64, ctypes.c_double, ctypes.POINTER(FrameInfo)],
+    })
     for name, args in sig.items():
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = ctypes.c_int
+    L.diver_server_clock.argtypes = []
+    L.diver_server_clock.restype = ctypes.c_double
+    L.diver_server_take_completed.argtypes = [vp, ctypes.POINTER(ctypes.c_int64), i64]
+    L.diver_server_take_completed.restype = ctypes.c_int64
+    L.diver_server_history.argtypes = [vp, ctypes.POINTER(FrameInfo), i64,
+                                       ctypes.POINTER(ctypes.c_int64)]
+    L.diver_server_history.restype = ctypes.c_int64
     L.diver_decoder_param_count.argtypes = [ctypes.POINTER(DecoderDesc)]
     L.diver_decoder_param_count.restype = ctypes.c_int64
 
@@ -133,6 +169,13 @@ def call(name: str, *args) -> None:
     st = getattr(L, name)(*args)
     if st != 0:
         raise DiverError(name, st, L.diver_last_error().decode(errors='replace'))
+
+
+def status_call(name: str, *args) -> tuple[int, str]:
+    """Calls an entry point and returns (status, diver_last_error() text if it failed)."""
+    L = lib()
+    st = getattr(L, name)(*args)
+    return st, (L.diver_last_error().decode(errors='replace') if st != 0 else '')
 
 
 def stream_of(t) -> int:

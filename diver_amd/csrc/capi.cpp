@@ -68,6 +68,14 @@ uint16_t to_f16(float f) {
 
 }  // namespace
 
+namespace diver {
+// Sets the calling thread's diver_last_error() text (used by the render server's entry points).
+int set_error(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+}  // namespace diver
+
 struct diver_decoder {
     diver_decoder_desc desc;
     int variant;

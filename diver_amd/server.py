@@ -1,19 +1,20 @@
-"""Continuous-batching render server on one GPU.
+"""Continuous-batching render server on one GPU: a Python face over the native serving loop
+(``diver_server_*`` in include/diver.h).
 
-Scenes (a SceneGrid plus a Decoder) stay resident in device memory.  Clients submit frame
-requests (scene, camera, resolution) from any thread; a render loop thread packs the pending
-requests of one scene (FIFO, up to ``max_requests`` requests and ``max_rays`` rays; frames above
-``split_rays`` rays apart from smaller ones) into one ``diver_render_batch`` launch and keeps up to
-``n_streams`` launches in flight.  Requests that
-arrive while launches run are admitted into the next launch as soon as a stream slot and memory
-are free, so their work overlaps the frames already on the GPU.  A second thread waits on the
-oldest launch's completion event (blocking, without polling) and releases its frames and its
-stream slot the moment the GPU finishes it.
+Scenes (a SceneGrid plus a Decoder) stay resident in device memory.  Clients submit frame requests
+(scene, camera, resolution) from any thread; the library's launch thread packs the pending frames
+of one scene (FIFO, up to ``max_requests`` frames and ``max_rays`` ray ids; frames above
+``split_rays`` pixels apart from smaller ones) into one ``diver_render_batch`` launch and keeps up
+to ``n_streams`` launches in flight on streams of its own.  Frames that arrive while launches run
+join the next launch as soon as a stream and memory are free, so their work overlaps the frames
+already on the GPU.  The library's retire thread watches every launch's completion event and
+completes its frames the moment the GPU finishes them.  Neither thread holds the GIL: a client's
+``result()`` blocks inside the library, and the per-launch host work is native.
 
-Memory admission: every resident scene and every in-flight output buffer is charged against
-``hbm_budget`` bytes.  A scene that does not fit is refused at ``add_scene``; a request whose
-output could never fit next to the resident scenes is refused at ``submit``; a request that fits
-only later waits in the queue until launches complete and release their outputs.
+Memory admission: every resident scene and the output buffers of the frames on the GPU are
+charged against ``hbm_budget`` bytes.  A scene that does not fit is refused at ``add_scene``; a
+frame whose output could never fit next to the resident scenes is refused at ``submit``; a frame
+that fits only later waits in the queue until launches complete.
 
 A ray's result depends only on its own intervals, so a frame rendered in a mixed launch is
 bit-identical to the same frame rendered alone.
@@ -24,10 +25,10 @@ eval_rt.py:106-135); this module is the serving path built around the same rende
 from __future__ import annotations
 
 import collections
+import ctypes
 import dataclasses
 import threading
-import time
-from typing import Deque, Dict, List, Optional
+from typing import Dict, Optional
 
 import numpy as np
 import torch
@@ -37,6 +38,11 @@ from .decoder import Decoder
 from .renderer import SceneGrid, focal_for
 
 _OUT_BYTES_PER_PIXEL = 16 + 4  # rgba fp32 + int32 interval count
+
+
+def clock() -> float:
+    """Seconds on the clock frames are stamped with (CLOCK_MONOTONIC, as time.perf_counter)."""
+    return _lib.lib().diver_server_clock()
 
 
 class AdmissionError(RuntimeError):
@@ -52,24 +58,29 @@ class ServedScene:
     traversal: str = 'auto'
     t_stop: float = 1e-4
     nbytes: int = 0
+    scene_id: int = -1
 
 
 class FrameRequest:
-    """One frame: filled in by the server; ``result()`` waits for it."""
+    """One frame; ``result()`` waits for it.  ``rgba`` [H, W, 4] and ``count`` [H, W] are the
+    device buffers the library renders into."""
 
-    def __init__(self, scene: str, c2w, width: int, height: int, fov: float):
+    def __init__(self, server: 'RenderServer', scene: str, ticket: int, width: int, height: int,
+                 rgba: torch.Tensor, count: torch.Tensor):
+        self._server = server
         self.scene = scene
-        self.c2w = np.asarray(c2w, np.float32).reshape(3, 4)
+        self.ticket = ticket
         self.width = int(width)
         self.height = int(height)
-        self.fov = float(fov)
-        self.t_submit = time.perf_counter()
+        self.rgba = rgba
+        self.count = count
+        self.t_submit: Optional[float] = None
         self.t_launch: Optional[float] = None
         self.t_done: Optional[float] = None
-        self.rgba: Optional[torch.Tensor] = None
-        self.count: Optional[torch.Tensor] = None
+        self.launch: Optional[int] = None
         self.error: Optional[BaseException] = None
-        self._done = threading.Event()
+        self._done = False
+        self._lock = threading.Lock()
 
     @property
     def rays(self) -> int:
@@ -84,23 +95,36 @@ class FrameRequest:
         return (self.t_done - self.t_submit) if self.t_done is not None else float('nan')
 
     def done(self) -> bool:
-        return self._done.is_set()
+        return self._done
+
+    def _wait(self, timeout: Optional[float]):
+        with self._lock:
+            if self._done:
+                return
+            info = _lib.FrameInfo()
+            st, msg = _lib.status_call('diver_server_wait', self._server._handle, self.ticket,
+                                       -1.0 if timeout is None else float(timeout),
+                                       ctypes.byref(info))
+            if st == _lib.DIVER_ETIMEDOUT:
+                raise TimeoutError('frame not rendered in time')
+            if st != 0 and info.ticket != self.ticket:
+                # the wait itself failed (no record of the frame)
+                self.error = _lib.DiverError('diver_server_wait', st, msg)
+            else:
+                self.t_submit, self.t_launch, self.t_done = info.t_submit, info.t_launch, info.t_done
+                self.launch = info.launch
+                if st != 0:
+                    self.error = _lib.DiverError('diver_server_wait', st, msg)
+            self._done = True
+            self._server._release(self.ticket)
 
     def result(self, timeout: Optional[float] = None):
         """(rgb with white background [H, W, 3], transmittance [H, W], count [H, W])."""
-        if not self._done.wait(timeout):
-            raise TimeoutError('frame not rendered in time')
+        self._wait(timeout)
         if self.error is not None:
             raise self.error
         rgba = self.rgba
         return rgba[..., :3] + rgba[..., 3:], rgba[..., 3], self.count
-
-
-@dataclasses.dataclass
-class _Launch:
-    event: torch.cuda.Event
-    requests: List[FrameRequest]
-    nbytes: int
 
 
 class RenderServer:
@@ -112,26 +136,18 @@ class RenderServer:
             free, _total = torch.cuda.mem_get_info(self.device)
             hbm_budget = int(free * 0.9)
         self.hbm_budget = int(hbm_budget)
-        self.max_requests = max(1, min(int(max_requests), _lib.MAX_REQUESTS))
-        self.max_rays = int(max_rays)
-        # frames above split_rays rays (512x512) and frames up to it go to separate launches, so
-        # a small frame never waits for a large one's launch to finish; None mixes them
-        self.split_rays = None if split_rays is None else int(split_rays)
-        self.streams = [torch.cuda.Stream(device=self.device) for _ in range(max(1, n_streams))]
+        cfg = _lib.ServerConfig(self.hbm_budget, int(n_streams),
+                                max(1, min(int(max_requests), _lib.MAX_REQUESTS)), int(max_rays),
+                                0 if split_rays is None else int(split_rays))
+        handle = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            _lib.call('diver_server_create', ctypes.byref(cfg), ctypes.byref(handle))
+        self._handle = handle.value
         self.scenes: Dict[str, ServedScene] = {}
-        self._resident = 0
-        self._inflight_bytes = 0
-        self._pending: Deque[FrameRequest] = collections.deque()
-        self._inflight: List[_Launch] = []
-        self._cv = threading.Condition()
-        self._stop = False
-        self._next_stream = 0
-        self.launches = 0
-        self.completed: List[FrameRequest] = []
-        self._thread = threading.Thread(target=self._loop, name='diver-render-loop', daemon=True)
-        self._retirer = threading.Thread(target=self._retire_loop, name='diver-retire', daemon=True)
-        self._thread.start()
-        self._retirer.start()
+        self._lock = threading.Lock()
+        # output buffers of frames the library may still write (dropped once complete)
+        self._live: Dict[int, FrameRequest] = {}
+        self._closed = False
 
     # ---- scenes ---------------------------------------------------------------------------
     def add_scene(self, name: str, grid: SceneGrid, decoder: Decoder, precision: str = 'bf16',
@@ -139,39 +155,94 @@ class RenderServer:
         nbytes = grid.nbytes + sum(t.numel() * t.element_size()
                                    for t in (grid.blocks, grid.bricks) if t is not None)
         nbytes += decoder.nbytes
-        with self._cv:
+        walk = ('realtime' if grid.octrees is not None else 'offline') \
+            if traversal == 'auto' else traversal
+        opts = calls.render_opts(precision, walk, t_stop, 0.0)
+        g = calls._grid_of(grid)
+        sid = ctypes.c_int(-1)
+        with self._lock:
             if name in self.scenes:
                 raise ValueError(f'scene {name!r} is already resident')
-            if self._resident + nbytes > self.hbm_budget:
-                raise AdmissionError(
-                    f'scene {name!r} needs {nbytes} bytes; {self.hbm_budget - self._resident} '
-                    f'of the {self.hbm_budget}-byte budget are free')
-            sc = ServedScene(name, grid, decoder, precision, traversal, t_stop, nbytes)
+            with torch.cuda.device(self.device):
+                # the grid's structures were built on the current stream: launches wait for them
+                st, msg = _lib.status_call('diver_server_add_scene', self._handle, ctypes.byref(g),
+                                           decoder.handle, ctypes.byref(opts), nbytes,
+                                           torch.cuda.current_stream(self.device).cuda_stream,
+                                           ctypes.byref(sid))
+            if st == _lib.DIVER_EBUDGET:
+                raise AdmissionError(f'scene {name!r}: {msg}')
+            if st != 0:
+                raise _lib.DiverError('diver_server_add_scene', st, msg)
+            sc = ServedScene(name, grid, decoder, precision, traversal, t_stop, nbytes, sid.value)
             self.scenes[name] = sc
-            self._resident += nbytes
         return sc
 
     # ---- requests -------------------------------------------------------------------------
     def submit(self, scene: str, c2w, width: int, height: int, fov: float = 0.69) -> FrameRequest:
-        req = FrameRequest(scene, c2w, width, height, fov)
-        with self._cv:
-            if scene not in self.scenes:
-                raise KeyError(f'unknown scene {scene!r}')
-            if self._resident + req.out_bytes > self.hbm_budget:
-                raise AdmissionError(f'a {width}x{height} frame needs {req.out_bytes} bytes of '
-                                     f'output; {self.hbm_budget - self._resident} are free')
-            if req.rays > self.max_rays:
-                raise AdmissionError(f'{req.rays} rays exceed the per-launch limit {self.max_rays}')
-            self._pending.append(req)
-            self._cv.notify_all()
+        sc = self.scenes.get(scene)
+        if sc is None:
+            raise KeyError(f'unknown scene {scene!r}')
+        c2w = np.ascontiguousarray(np.asarray(c2w, np.float32).reshape(3, 4))
+        with torch.cuda.device(self.device):
+            rgba = torch.empty((height, width, 4), dtype=torch.float32, device=self.device)
+            count = torch.empty((height, width), dtype=torch.int32, device=self.device)
+            ticket = ctypes.c_int64(0)
+            # the buffers' memory may have been used by work still queued on the current stream
+            st, msg = _lib.status_call(
+                'diver_server_submit', self._handle, sc.scene_id,
+                c2w.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), int(width), int(height),
+                float(focal_for(width, fov)), rgba.data_ptr(), count.data_ptr(),
+                torch.cuda.current_stream(self.device).cuda_stream, ctypes.byref(ticket))
+        if st == _lib.DIVER_EBUDGET:
+            raise AdmissionError(msg)
+        if st != 0:
+            raise _lib.DiverError('diver_server_submit', st, msg)
+        req = FrameRequest(self, scene, ticket.value, width, height, rgba, count)
+        with self._lock:
+            self._live[req.ticket] = req
+        self._reap()
         return req
 
+    def _release(self, ticket: int):
+        with self._lock:
+            self._live.pop(ticket, None)
+
+    def _reap(self):
+        """Drops the server's references to frames the library has completed (the clients keep
+        theirs)."""
+        buf = (ctypes.c_int64 * 256)()
+        while True:
+            n = _lib.lib().diver_server_take_completed(self._handle, buf, 256)
+            with self._lock:
+                for i in range(n):
+                    self._live.pop(buf[i], None)
+            if n < 256:
+                break
+
+    @property
+    def launches(self) -> int:
+        n = ctypes.c_int64(0)
+        _lib.lib().diver_server_history(self._handle, None, 0, ctypes.byref(n))
+        return n.value
+
     def close(self):
-        with self._cv:
-            self._stop = True
-            self._cv.notify_all()
-        self._thread.join()
-        self._retirer.join()
+        if self._closed:
+            return
+        self._closed = True
+        # every frame already submitted is rendered; its record is read before the library's
+        # state goes away, so result() still answers after close()
+        with self._lock:
+            live = list(self._live.values())
+        for req in live:
+            try:
+                req._wait(None)
+            except Exception:  # noqa: BLE001 -- the error stays on the request
+                pass
+        st, msg = _lib.status_call('diver_server_destroy', self._handle)
+        self._handle = None
+        self._live.clear()
+        if st != 0:
+            raise _lib.DiverError('diver_server_destroy', st, msg)
 
     def __enter__(self):
         return self
@@ -179,113 +250,26 @@ class RenderServer:
     def __exit__(self, *exc):
         self.close()
 
-    # ---- render loop ----------------------------------------------------------------------
-    def _size_class(self, req: FrameRequest) -> bool:
-        return self.split_rays is not None and req.rays > self.split_rays
-
-    def _take_batch(self) -> List[FrameRequest]:
-        """Pending requests of the oldest request's scene and size class that fit the launch and
-        memory limits (FIFO within the scene).  Called with the lock held."""
-        if not self._pending:
-            return []
-        scene = self._pending[0].scene
-        size_class = self._size_class(self._pending[0])
-        free = self.hbm_budget - self._resident - self._inflight_bytes
-        batch, rays, nbytes = [], 0, 0
-        for req in list(self._pending):
-            if req.scene != scene or self._size_class(req) != size_class:
-                continue
-            if (len(batch) == self.max_requests or rays + req.rays > self.max_rays or
-                    nbytes + req.out_bytes > free):
-                break
-            batch.append(req)
-            rays += req.rays
-            nbytes += req.out_bytes
-        for req in batch:
-            self._pending.remove(req)
-        return batch
-
-    def _finished(self):
-        """Nothing pending or in flight after close().  Called with the lock held."""
-        return self._stop and not self._pending and not self._inflight
-
-    def _launch(self, batch: List[FrameRequest]):
-        sc = self.scenes[batch[0].scene]
-        stream = self.streams[self._next_stream % len(self.streams)]
-        self._next_stream += 1
-        t = time.perf_counter()
-        traversal = ('realtime' if sc.grid.octrees is not None else 'offline') \
-            if sc.traversal == 'auto' else sc.traversal
-        opts = calls.render_opts(sc.precision, traversal, sc.t_stop, 0.0)
-        reqs = [dict(c2w=r.c2w, width=r.width, height=r.height, focal=focal_for(r.width, r.fov))
-                for r in batch]
-        with torch.cuda.stream(stream):
-            try:
-                outs = calls.render_batch(sc.grid, sc.decoder, reqs, opts)
-            except Exception as e:  # a refused launch fails its requests, not the server
-                for r in batch:
-                    r.error = e
-                    r.t_done = time.perf_counter()
-                    r._done.set()
-                return
-            ev = torch.cuda.Event()
-            ev.record(stream)
-        for r, (rgba, count) in zip(batch, outs):
-            r.rgba, r.count, r.t_launch = rgba, count, t
-        nbytes = sum(r.out_bytes for r in batch)
-        self._inflight.append(_Launch(ev, batch, nbytes))
-        self._inflight_bytes += nbytes
-        self.launches += 1
-        self._cv.notify_all()  # the retire thread waits for a launch to watch
-
-    def _loop(self):
-        """Launch thread: packs admissible requests whenever a stream slot is free."""
-        torch.cuda.set_device(self.device)
-        with self._cv:
-            while not self._finished():
-                batch = self._take_batch() if len(self._inflight) < len(self.streams) else []
-                if batch:
-                    self._launch(batch)
-                else:
-                    # woken by an arrival, a completed launch (slot and memory freed) or close()
-                    self._cv.wait()
-
-    def _retire_loop(self):
-        """Retire thread: blocks on the oldest launch's event (the GIL is released while it
-        waits), then hands its frames to their clients and frees its slot and memory."""
-        torch.cuda.set_device(self.device)
-        while True:
-            with self._cv:
-                while not self._inflight:
-                    if self._finished():
-                        return
-                    self._cv.wait()
-                launch = self._inflight[0]
-            launch.event.synchronize()
-            now = time.perf_counter()
-            with self._cv:
-                self._inflight.remove(launch)
-                self._inflight_bytes -= launch.nbytes
-                for req in launch.requests:
-                    req.t_done = now
-                    self.completed.append(req)
-                    req._done.set()
-                self._cv.notify_all()
-
     # ---- statistics -----------------------------------------------------------------------
+    def history(self):
+        """Records of the last completed frames (up to 65536), oldest first."""
+        buf = (_lib.FrameInfo * 65536)()
+        n = _lib.lib().diver_server_history(self._handle, buf, 65536, None)
+        return [buf[i] for i in range(n)]
+
     def stats(self, since: float = 0.0) -> dict:
-        reqs = [r for r in self.completed if r.t_submit >= since and r.error is None]
-        if not reqs:
+        recs = [r for r in self.history() if r.t_submit >= since and r.status == 0]
+        if not recs:
             return {'frames': 0}
-        lat = np.array([r.latency for r in reqs]) * 1e3
-        t0 = min(r.t_submit for r in reqs)
-        t1 = max(r.t_done for r in reqs)
-        rays = sum(r.rays for r in reqs)
+        lat = np.array([r.t_done - r.t_submit for r in recs]) * 1e3
+        t0 = min(r.t_submit for r in recs)
+        t1 = max(r.t_done for r in recs)
+        rays = sum(r.width * r.height for r in recs)
         by_res = collections.defaultdict(list)
-        for r in reqs:
-            by_res[f'{r.width}x{r.height}'].append(r.latency * 1e3)
+        for r, ms in zip(recs, lat):
+            by_res[f'{r.width}x{r.height}'].append(ms)
         return {
-            'frames': len(reqs), 'launches': self.launches, 'rays': rays,
+            'frames': len(recs), 'launches': self.launches, 'rays': rays,
             'wall_s': t1 - t0, 'rays_per_s': rays / max(t1 - t0, 1e-9),
             'p50_ms': float(np.percentile(lat, 50)), 'p99_ms': float(np.percentile(lat, 99)),
             'by_resolution': {k: {'frames': len(v), 'p50_ms': float(np.percentile(v, 50)),

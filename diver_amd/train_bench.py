@@ -6,8 +6,9 @@ reference's autograd structure (stored samples), on one GPU.
 Scene: the 256^3 x 32-channel sphere/shell grid with trainable corner features and a benchmark
 decoder of hidden width 64 (density softplus(F[0]); colour MLP (31 + 27) -> 64 -> 64 -> 3).
 Each step draws 8192 random rays from the 200 hemisphere cameras (800x800), renders them, takes
-the MSE against colours rendered from a second seeded grid, and runs Adam on features and
-decoder.  Both paths composite every interval (no early termination, as the reference's offline
+the reference's loss -- MSE against colours rendered from a second seeded grid plus
+l_s * mean(log(1 + sigma^2 / 0.5)) over the [B, M] raw densities, l_s = 1e-5 (train.py:84-85,
+109-113) -- and runs Adam on features and decoder.  Both paths composite every interval (no early termination, as the reference's offline
 training forward, model/diver.py:183-207).
 
   recompute: FusedRenderer -- traversal, integral, decoder and compositing in one kernel; the
@@ -43,8 +44,11 @@ from .utils.integrator import integrate
 from .utils.ray_voxel_intersection import masked_intersect
 
 
+L_S = 1e-5  # the reference's default sparsity weight (configs/*.py 'l_s')
+
+
 def stored_forward(voxels, dec, mask, n, o, d):
-    """Reference-structured forward with autograd over stored samples."""
+    """Reference-structured forward with autograd over stored samples: (rgb, sigma_loss)."""
     xyzmin, vs = -1.0, 2.0 / n
     coord, hit, _tns = masked_intersect(o, d, xyzmin, 1.0, float(n), vs, mask, 1.0)
     rows = coord[hit]
@@ -60,7 +64,8 @@ def stored_forward(voxels, dec, mask, n, o, d):
                                                                  col)
     # composite() applies relu to sigma (model/diver.py:195); softplus output is already >= 0
     rgb, _w = composite(sigma, color, hit, True)
-    return rgb
+    raw = torch.zeros(B, M, device=o.device).masked_scatter(hit, F[:, 0])
+    return rgb, torch.log(1 + raw ** 2 / 0.5).mean()
 
 
 def run(path, steps, warmup, n_rays, dev, optimizer='diver'):
@@ -86,7 +91,7 @@ def run(path, steps, warmup, n_rays, dev, optimizer='diver'):
     if path == 'recompute':
         model = FusedRenderer(grid, dec, traversal='offline')
         params = list(model.parameters())
-        fwd = lambda o, d: model(o, d, 0.0)
+        fwd = lambda o, d: model(o, d, 0.0, sparsity=True)
     else:
         voxels = torch.nn.Parameter(grid.voxels.clone())
         del grid
@@ -112,7 +117,8 @@ def run(path, steps, warmup, n_rays, dev, optimizer='diver'):
         if timed:
             e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
             e[0].record()
-        loss = torch.mean((fwd(og, dg) - tg) ** 2)
+        rgb, sigma_loss = fwd(og, dg)
+        loss = torch.mean((rgb - tg) ** 2) + L_S * sigma_loss
         loss.backward()
         if timed:
             e[1].record()
@@ -159,7 +165,8 @@ def main(argv=None):
     args = ap.parse_args(argv)
     dev = torch.device('cuda', 0)
     out = {'workload': 'training step, 256^3 x 32 trainable grid, hidden-64 benchmark decoder, '
-                       f'{args.rays} random rays, MSE + Adam, no early termination',
+                       f'{args.rays} random rays, MSE + 1e-5 sparsity loss, Adam, no early '
+                       'termination',
            'steps': args.steps, 'optimizer': args.optimizer}
     paths = ('recompute', 'stored') if args.path == 'both' else (args.path,)
     for path in paths:

@@ -28,7 +28,10 @@ enum diver_status {
     DIVER_EINVAL = 1,       /* bad argument (null pointer, size, unsupported shape)          */
     DIVER_ENOMEM = 2,       /* device allocation failed                                       */
     DIVER_EHIP = 3,         /* HIP runtime error (launch or copy)                             */
-    DIVER_EUNSUPPORTED = 4  /* combination not compiled into the library                      */
+    DIVER_EUNSUPPORTED = 4, /* combination not compiled into the library                      */
+    DIVER_EBUDGET = 5,      /* a render server's memory budget can never hold the scene/frame    */
+    DIVER_ETIMEDOUT = 6,    /* a wait ended before the frame was rendered                        */
+    DIVER_ESTATE = 7        /* the server has failed or is closing; the frame was not rendered    */
 };
 
 const char* diver_version(void);
@@ -334,6 +337,78 @@ int diver_walk_rays(const diver_grid* grid, int traversal, int64_t n_rays, const
 /* The FP32 rays diver_render_camera traces: o, d [height*width][3]. */
 int diver_camera_rays(const float* c2w_host, int width, int height, float focal, float* o,
                       float* d, diver_stream_t stream);
+
+/* ---- render server ---------------------------------------------------------------------------- */
+
+/* A continuous-batching render loop on one GPU, run by two native threads.  Scenes (a grid plus a
+ * decoder and render options) stay resident; clients submit frames (scene, pinhole camera, output
+ * buffers) from any thread.  The launch thread packs the pending frames of the oldest frame's
+ * scene and size class (FIFO; up to max_requests frames and max_rays ray ids; frames of more than
+ * split_rays pixels launch apart from smaller ones) into one diver_render_batch launch, with up to
+ * n_streams launches in flight on streams of its own.  Frames that arrive while launches run join
+ * the next launch as soon as a stream and memory are free.  The retire thread watches every
+ * launch's completion event and completes its frames as soon as the GPU finishes them, in
+ * whatever order the launches end.
+ * Memory admission: resident scenes (their scene_bytes) and the outputs of the frames on the GPU
+ * (20 bytes per pixel: rgba + count) are charged against hbm_budget; a scene or a frame that can
+ * never fit is refused with DIVER_EBUDGET, a frame that fits only later waits.
+ * A frame's pixels equal diver_render_camera's for the same scene and camera, bit for bit.
+ * No counterpart in the reference (its real-time viewer renders one camera at a time,
+ * eval_rt.py:106-135); this is the serving path around the same renderer. */
+typedef struct diver_server diver_server;
+
+typedef struct diver_server_config {
+    int64_t hbm_budget;  /* bytes for resident scenes plus the outputs of frames on the GPU     */
+    int n_streams;       /* launches in flight, 1..8                                           */
+    int max_requests;    /* frames per launch, 1..DIVER_MAX_REQUESTS                            */
+    int64_t max_rays;    /* ray ids per launch (a camera takes 64 per 8x8 tile)                 */
+    int64_t split_rays;  /* frames of more pixels launch apart from smaller ones; 0 mixes them  */
+} diver_server_config;
+
+typedef struct diver_frame_info {
+    int64_t ticket;
+    int scene;
+    int width;
+    int height;
+    int status;          /* DIVER_OK once rendered; an error code if its launch failed          */
+    int64_t launch;      /* serial number of the launch that rendered it (-1 before launch)    */
+    double t_submit;     /* diver_server_clock() at submit, at launch, at completion (seconds)  */
+    double t_launch;
+    double t_done;
+} diver_frame_info;
+
+/* Seconds on the monotonic clock the server stamps frames with (CLOCK_MONOTONIC). */
+double diver_server_clock(void);
+/* Creates the server on the calling thread's current device and starts its two threads. */
+int diver_server_create(const diver_server_config* cfg, diver_server** out);
+/* Renders every frame already submitted, then stops the threads and frees the server. */
+int diver_server_destroy(diver_server* srv);
+/* Makes a scene resident (the grid's and decoder's device memory stay the caller's and must
+ * outlive the server).  Work the caller enqueued on `ready` before this call (building the grid)
+ * is ordered before every launch of the scene.  Needs the block and brick bits and a decoder
+ * without pre-layers.  *scene_id receives the scene's number. */
+int diver_server_add_scene(diver_server* srv, const diver_grid* grid, const diver_decoder* dec,
+                           const diver_render_opts* opts, int64_t scene_bytes,
+                           diver_stream_t ready, int* scene_id);
+/* Queues one frame: c2w_host row-major 3x4 (copied), rgba [height][width][4] and count
+ * [height][width] (optional) device buffers that the caller keeps alive until the frame is
+ * complete.  Work the caller enqueued on `after` before this call (e.g. earlier use of the
+ * output memory) is ordered before the frame's launch; NULL orders nothing.  Returns at once
+ * with *ticket. */
+int diver_server_submit(diver_server* srv, int scene_id, const float* c2w_host, int width,
+                        int height, float focal, float* rgba, int32_t* count,
+                        diver_stream_t after, int64_t* ticket);
+/* Blocks until the frame is complete (or timeout_s seconds pass: DIVER_ETIMEDOUT; timeout_s < 0
+ * waits without limit) and fills *info (may be NULL).  A completed frame is waited for at most
+ * once. */
+int diver_server_wait(diver_server* srv, int64_t ticket, double timeout_s, diver_frame_info* info);
+/* Tickets of the frames completed since the previous call (at most max), oldest first; returns
+ * how many were written. */
+int64_t diver_server_take_completed(diver_server* srv, int64_t* tickets, int64_t max);
+/* The records of the last completed frames (at most max, oldest first, up to 65536 kept) and the
+ * number of launches so far; returns how many records were written. */
+int64_t diver_server_history(diver_server* srv, diver_frame_info* out, int64_t max,
+                             int64_t* launches);
 
 #ifdef __cplusplus
 }
