@@ -1,10 +1,15 @@
-"""Benchmark: rays/s and 800x800 frame latency of the fused render path (BF16 matrix-core decoder)
-against the plain FP32 GPU path (the reference's real-time op sequence on libdiver), on the
-256^3 / 32-channel synthetic scene (12 spheres + 4 shells, ~5% occupied), benchmark DIVeR32
-decoder, cameras on the upper hemisphere at radius 4, early termination at T < 1e-4.
+"""Benchmark: rays/s and 800x800 frame latency of the BF16 serving path (the native render server
+packing frames into fused launches, BF16 matrix-core decoder) against the plain FP32 GPU path (the
+reference's real-time op sequence on libdiver, one frame at a time), on the 256^3 / 32-channel
+synthetic scene (12 spheres + 4 shells, ~5% occupied), benchmark DIVeR32 decoder, cameras on the
+upper hemisphere at radius 4, early termination at T < 1e-4.
 
 Prints one JSON line.  A "step" is one 800x800 frame (640,000 rays) rendered from a resident
-scene; inputs (grid, mask, decoder) are in HBM before timing starts.
+scene; inputs (grid, mask, decoder) are in HBM before timing starts.  ``value`` is the server's
+throughput with all ``steps`` frames submitted at once (saturated); ``p50_ms`` / ``p99_ms`` are its
+submit-to-done frame latencies under Poisson arrivals at ``--rate`` frames/s.  Beside them: the
+fused kernel one frame at a time (its launch durations give the roofline), the FP32 decoder, the
+plain FP32 path, and the FP64-oracle accuracy of each.
 """
 from __future__ import annotations
 
@@ -32,20 +37,135 @@ def measured_traffic(n, width, height, channels):
         return json.load(f)['bf16']['traffic_bytes_per_launch']
 
 
+COUNTER_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'profiles',
+                            'render_800x800_256cube_counters.json')
+
+
+def counter_fractions(n, width, height, channels):
+    """Issue and matrix-core fractions of the render kernel from the committed rocprofv3 counter
+    passes (profiles/render_800x800_256cube_counters.json), and what limits the kernel."""
+    if (n, width, height, channels) != (256, 800, 800, 32) or not os.path.exists(COUNTER_FILE):
+        return {}
+    with open(COUNTER_FILE) as f:
+        c = json.load(f)['bf16']
+    return {k: c[k] for k in ('limiter', 'valu_issue_frac', 'issue_frac', 'mfma_frac',
+                              'mfma_tflops', 'instructions_per_voxel_step') if k in c}
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def serving_keys(serve, args, R, lat16):
+    """The line's latency keys: the server's at the offered rate, or (without the serving legs)
+    the kernel's one frame at a time."""
+    import numpy as np
+    if serve is None:
+        return {'p50_ms': round(float(np.percentile(lat16, 50)), 4),
+                'p99_ms': round(float(np.percentile(lat16, 99)), 4)}
+    lat, sat = serve['rate_lat'], serve['saturated_lat']
+    return {
+        'p50_ms': round(float(np.percentile(lat, 50)), 4),
+        'p99_ms': round(float(np.percentile(lat, 99)), 4),
+        'latency_offered_hz': args.rate,
+        'latency_offered_rays_s': round(args.rate * R, 1),
+        'latency_frames': serve['offered_frames'],
+        'latency_delivered_hz': round(serve['offered_frames'] / serve['rate_wall_s'], 1),
+        'saturated_p50_ms': round(float(np.percentile(sat, 50)), 4),
+        'saturated_p99_ms': round(float(np.percentile(sat, 99)), 4),
+        'serving_launches': serve['launches'],
+    }
+
+
+def serving_legs(args, grid, dec, cams, W, H, t_stop):
+    """The render server (two launches in flight) on 800x800 frames of the scene: (1) saturated:
+    ``steps`` frames submitted at once, wall clock from the first submit to the last result (the
+    timed region; GPU synchronised on both sides); (2) Poisson arrivals at ``rate`` frames/s for
+    ``rate_seconds`` after 0.3 s of untimed arrivals, a collector thread taking each result as it
+    is rendered.  Returns the wall time, per-frame latencies of both legs and launch counts."""
+    import collections
+    import threading
+
+    import numpy as np
+    import torch
+
+    from diver_amd import synthetic as S
+    from diver_amd.server import RenderServer
+
+    log('[bench] serving: saturated (timed region), then Poisson arrivals')
+    out = {}
+    with RenderServer(grid.voxels.device, n_streams=2) as srv:
+        srv.add_scene('s', grid, dec, precision='bf16', t_stop=t_stop)
+        for r in [srv.submit('s', cams[i % len(cams)], W, H) for i in range(args.warmup)]:
+            r.result(timeout=60)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        reqs = [srv.submit('s', cams[i % len(cams)], W, H) for i in range(args.steps)]
+        for r in reqs:
+            r.result(timeout=120)
+        torch.cuda.synchronize()
+        out['wall_s'] = time.perf_counter() - t0
+        out['launches'] = len({r.launch for r in reqs})
+        out['saturated_lat'] = [r.latency * 1e3 for r in reqs]
+        del reqs
+        rng = np.random.default_rng(S.SEED_ARRIVALS)
+        q = collections.deque()
+        done = threading.Event()
+        lat = []
+
+        def collect():
+            while not (done.is_set() and not q):
+                if not q:
+                    time.sleep(50e-6)
+                    continue
+                r = q.popleft()
+                r.result(timeout=120)
+                lat.append(r.latency * 1e3)
+
+        def arrivals(seconds):
+            t_start = time.perf_counter()
+            t_next = t_start
+            k = 0
+            while True:
+                t_next += rng.exponential(1.0 / args.rate)
+                if t_next - t_start > seconds:
+                    break
+                now = time.perf_counter()
+                if t_next > now:
+                    time.sleep(t_next - now)
+                q.append(srv.submit('s', cams[k % len(cams)], W, H))
+                k += 1
+            return k, time.perf_counter() - t_start
+
+        th = threading.Thread(target=collect)
+        th.start()
+        warm, _ = arrivals(0.3)  # untimed: the clients' output buffers reach their steady state
+        while len(lat) < warm:
+            time.sleep(1e-3)
+        lat.clear()
+        out['offered_frames'], out['rate_wall_s'] = arrivals(args.rate_seconds)
+        done.set()
+        th.join()
+        out['rate_lat'] = lat
+    return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=20)
-    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--steps', type=int, default=40)
+    ap.add_argument('--warmup', type=int, default=6)
+    ap.add_argument('--rate', type=float, default=2400.0,
+                    help='offered frames/s for the serving latency leg')
+    ap.add_argument('--rate-seconds', type=float, default=1.0)
     ap.add_argument('--res', type=int, default=800)
     ap.add_argument('--n', type=int, default=256)
     ap.add_argument('--plain-frames', type=int, default=5)
     ap.add_argument('--cpu-rays', type=int, default=16384)
     ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-serve', action='store_true',
+                    help='skip the serving legs (kernel profiling runs); value is then the '
+                         'one-frame-per-launch throughput')
     ap.add_argument('--calib-copy-mb', type=int, default=0,
                     help='copy this many MiB device-to-device first (counter calibration)')
     args = ap.parse_args()
@@ -124,7 +244,7 @@ def main():
     for i in range(args.warmup):
         fused('fp32')(i)
     lat32, _ = timed(fused('fp32'), args.steps)
-    log('[bench] fused BF16 (timed region)')
+    log('[bench] fused BF16, one frame per launch (kernel launch durations)')
     for i in range(args.warmup):
         fused('bf16')(i)
     counts.clear()
@@ -134,9 +254,8 @@ def main():
     wall = time.perf_counter() - t0
     i_eval = float(sum(int(c.sum()) for c in counts.values())) / args.steps
 
-    # ---- the same frames with two in flight on two streams (the render server's launch pattern):
-    # the central tiles' tail of one frame overlaps the next frame's start.  Reported beside the
-    # headline, which times one frame after another on one stream.
+    # ---- the same frames with two in flight on two streams, launched from Python: the central
+    # tiles' tail of one frame overlaps the next frame's start (what the server does natively)
     streams = [torch.cuda.Stream(device=dev), torch.cuda.Stream(device=dev)]
     for i in range(max(2, args.warmup)):
         with torch.cuda.stream(streams[i % 2]):
@@ -149,12 +268,20 @@ def main():
     torch.cuda.synchronize()
     two_stream_rays_s = R * args.steps / (time.perf_counter() - t0)
 
-    ms_per_step = float(np.mean(lat16))
-    value = R * args.steps / wall
+    # ---- serving path: the native render server, frames of 800x800 -------------------------
+    serve = None if args.no_serve else serving_legs(args, grid, dec, cams, W, H, t_stop)
+    frame_rays_s = R * args.steps / wall
+    if serve is not None:
+        ms_per_step = serve['wall_s'] * 1e3 / args.steps
+        value = R * args.steps / serve['wall_s']
+    else:
+        ms_per_step = wall * 1e3 / args.steps
+        value = frame_rays_s
     rays_fp32 = R / (np.mean(lat32) / 1e3)
     C = 32
     b_alg = R * 40 + i_eval * 8 * C * 4
-    achieved = b_alg / (ms_per_step / 1e3) / 1e9
+    launch_ms = float(np.mean(lat16))
+    achieved = b_alg / (launch_ms / 1e3) / 1e9
 
     # ---- accuracy on a sample of one frame: BF16 / FP32 GPU vs FP64 oracle ---------------------
     result_extra = {}
@@ -163,25 +290,32 @@ def main():
         import oracle
         from oracle import cpu_path, fp64
         from oracle.render import render as oracle_render
-        cam = cams[0]
-        o_all, d_all = camera_rays(cam, W, H, 0.69, dev)
-        stride = max(1, R // args.cpu_rays)
-        sel = torch.arange(0, R, stride, device=dev)[:args.cpu_rays]
-        o_s, d_s = o_all[sel].cpu().numpy(), d_all[sel].cpu().numpy()
-        rgb16, _, _ = render_camera(grid, dec, cam, W, H, precision='bf16', t_stop=t_stop)
-        rgb32, _, _ = render_camera(grid, dec, cam, W, H, precision='fp32', t_stop=t_stop)
-        rgb16 = rgb16.reshape(-1, 3)[sel].cpu().numpy()
-        rgb32 = rgb32.reshape(-1, 3)[sel].cpu().numpy()
-        img_plain, _ = rt.render(cam, W, H)
-        img_p = img_plain.reshape(-1, 3)[sel].cpu().numpy()
+        # the 4 views of the precision report (0, 50, 100, 150), strided pixels of each
+        views = [0, 50, 100, 150]
+        per_view = max(1, args.cpu_rays // len(views))
+        stride = max(1, R // per_view)
+        sel = torch.arange(0, R, stride, device=dev)[:per_view]
+        o_l, d_l, g16, g32, gp = [], [], [], [], []
+        for v in views:
+            cam = cams[v]
+            o_all, d_all = camera_rays(cam, W, H, 0.69, dev)
+            o_l.append(o_all[sel].cpu().numpy())
+            d_l.append(d_all[sel].cpu().numpy())
+            g16.append(render_camera(grid, dec, cam, W, H, precision='bf16',
+                                     t_stop=t_stop)[0].reshape(-1, 3)[sel].cpu().numpy())
+            g32.append(render_camera(grid, dec, cam, W, H, precision='fp32',
+                                     t_stop=t_stop)[0].reshape(-1, 3)[sel].cpu().numpy())
+            gp.append(rt.render(cam, W, H)[0].reshape(-1, 3)[sel].cpu().numpy())
+        o_s, d_s = np.concatenate(o_l), np.concatenate(d_l)
+        rgb16, rgb32, img_p = np.concatenate(g16), np.concatenate(g32), np.concatenate(gp)
         vox_np = sc.dense_voxels()
         ref = oracle_render(vox_np, sc.mask, sc.n, sc.xyzmin, sc.voxel_size,
                             lambda F, dd: fp64.baseline_decoder(layers, F, dd), 1, o_s, d_s,
                             t_stop, walk='realtime')
         rgb_ref = ref['rgb'] + ref['T'][:, None]
 
-        def psnr(a):
-            mse = float(np.mean((a.astype(np.float64) - rgb_ref) ** 2))
+        def psnr(a, want=rgb_ref):
+            mse = float(np.mean((a.astype(np.float64) - want) ** 2))
             return round(-10 * np.log10(max(mse, 1e-30)), 2)
         result_extra['psnr_vs_fp64'] = {'bf16': psnr(rgb16), 'fp32': psnr(rgb32),
                                         'plain_fp32': psnr(img_p)}
@@ -201,9 +335,25 @@ def main():
         cpu_s = time.perf_counter() - t0
         cpu = {'value': round(o_s.shape[0] / cpu_s, 1), 'unit': 'rays/s', 'cores': ncpu,
                'kind': 'port',
-               'sample': f'{o_s.shape[0]} strided pixels of one {W}x{H} view, offline path '
-                         '(traverse, integrate, nn.Linear decoder, cumprod composite), FP32, '
-                         'no early termination'}
+               'sample': f'{o_s.shape[0]} strided pixels of the {W}x{H} views {views} (the '
+                         'FP64-oracle views), offline path (traverse, integrate, nn.Linear '
+                         'decoder, cumprod composite), FP32, no early termination'}
+        # the CPU configuration in full: 64^3 x 32, 6 spheres, one 64x64 view (4096 rays),
+        # median of 20 runs after 3 warm-ups
+        sc1 = S.config1_scene()
+        o1, d1 = S.camera_rays(cams[0], 64, 64)
+        vox1 = torch.from_numpy(sc1.dense_voxels())
+        runs = []
+        for i in range(23):
+            t1 = time.perf_counter()
+            cpu_path.render(o1, d1, vox1, sc1.mask, sc1.n, sc1.xyzmin, sc1.voxel_size, decode,
+                            act)
+            runs.append(time.perf_counter() - t1)
+        med = float(np.median(runs[3:]))
+        result_extra['cpu_config1'] = {'rays_per_s': round(4096 / med, 1),
+                                       'ms_per_frame': round(med * 1e3, 2), 'cores': ncpu,
+                                       'runs': 20, 'sample': '64^3 x 32, 6 spheres, one 64x64 '
+                                                             'view, 4096 rays, FP32'}
         result_extra['psnr_vs_fp64']['cpu_fp32'] = psnr(cpu_rgb.numpy())
 
     line = {
@@ -219,22 +369,31 @@ def main():
         'vs_baseline': None,
         'dtype': 'bf16 decoder (fp32 traversal/integral/transmittance)',
         'data': 'synthetic (seeded spheres+shells grid, Kaiming decoder, hemisphere cameras)',
-        'config': {'workload': f'fused render, {W}x{H} frames, {args.n}^3 grid, 32 channels, '
+        'config': {'workload': ('serving' if serve else 'fused render, one launch per frame:')
+                               + f' {W}x{H} frames (BF16 decoder), {args.n}^3 grid, 32 channels, '
                                'DIVeR32 benchmark decoder, T<1e-4 termination',
                    'frames': args.steps, 'rays_per_frame': R,
                    'occupancy': round(sc.occupancy, 4)},
-        'p50_ms': round(float(np.percentile(lat16, 50)), 4),
-        'p99_ms': round(float(np.percentile(lat16, 99)), 4),
+        **serving_keys(serve, args, R, lat16),
+        'single_frame_rays_s': round(frame_rays_s, 1),
+        'single_frame_p50_ms': round(float(np.percentile(lat16, 50)), 4),
+        'single_frame_p99_ms': round(float(np.percentile(lat16, 99)), 4),
         'fused_fp32_rays_s': round(rays_fp32, 1),
         'plain_fp32_rays_s': round(plain_rays_s, 1),
         'plain_fp32_p50_ms': round(float(np.percentile(plain_lat, 50)), 3),
+        'plain_fp32_p99_ms': round(float(np.percentile(plain_lat, 99)), 3),
         'plain_rounds_per_frame': rounds,
         'speedup_vs_plain_fp32': round(value / plain_rays_s, 2),
+        'single_frame_speedup_vs_plain_fp32': round(frame_rays_s / plain_rays_s, 2),
         'two_frames_in_flight_rays_s': round(two_stream_rays_s, 1),
         'intervals_per_frame': i_eval,
+        # the render kernel, one 800x800 frame per launch: algorithmic bytes over its event-timed
+        # launch duration, priced on the HBM roofline; 'limiter' is what bounds it instead
         'roofline': {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBS,
                      'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4),
-                     'traffic': measured_traffic(args.n, W, H, C)},
+                     'traffic': measured_traffic(args.n, W, H, C),
+                     'kernel': 'render_tile_kernel', 'launch_ms': round(launch_ms, 4),
+                     **counter_fractions(args.n, W, H, C)},
         'cpu_baseline': cpu,
     }
     line.update(result_extra)

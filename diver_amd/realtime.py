@@ -48,6 +48,28 @@ class RealTimeRenderer:
         self.octrees = build_octrees(m).to(torch.uint8)
         self._buffers = {}
 
+    @classmethod
+    def from_serialized(cls, scene, xyzmin: float, voxel_size: float, alpha_skip: float = 1e-2,
+                        t_stop: float = 1e-2) -> 'RealTimeRenderer':
+        """From a loaded ``serialize.pth`` (diver_amd.checkpoint.load_serialized): its rows, corner
+        map and blob decoder as eval_rt.py:52-89 installs them."""
+        self = cls.__new__(cls)
+        n = scene.n
+        if n % 64:
+            raise ValueError('the octree descent needs N to be a multiple of 64')
+        self.n = n
+        self.xyzmin = float(xyzmin)
+        self.voxel_size = float(voxel_size)
+        self.decoder = Decoder(scene.rt_decoder())
+        self.alpha_skip = alpha_skip
+        self.t_stop = t_stop
+        self.mask = scene.mask.to(torch.uint8).contiguous()
+        self.features = scene.features
+        self.voxel_map = scene.voxel_map.reshape(n + 1, n + 1, n + 1)
+        self.octrees = build_octrees(scene.mask.bool()).to(torch.uint8)
+        self._buffers = {}
+        return self
+
     def _bufs(self, height, width, dev):
         key = (height, width)
         if key not in self._buffers:
