@@ -54,13 +54,15 @@ __device__ __forceinline__ bool box_entry(float ox, float oy, float oz, float dx
 }
 
 // One DDA step: next voxel-boundary crossing from (px,py,pz), snapped onto the crossed plane.
+// The reference's floor(p * s + 1) * s (s = -1, 0 or 1) is floor(fma(p, s, 1)) * s here: p * s is
+// exact, so the fused form rounds once, as the reference's add does.
 __device__ __forceinline__ float dda_step(float px, float py, float pz, float dx, float dy, float dz,
                                           float sx, float sy, float sz, float& qx, float& qy,
                                           float& qz) {
 #pragma clang fp contract(off)
-    float bx = floorf(px * sx + 1.0f) * sx;
-    float by = floorf(py * sy + 1.0f) * sy;
-    float bz = floorf(pz * sz + 1.0f) * sz;
+    float bx = floorf(__builtin_fmaf(px, sx, 1.0f)) * sx;
+    float by = floorf(__builtin_fmaf(py, sy, 1.0f)) * sy;
+    float bz = floorf(__builtin_fmaf(pz, sz, 1.0f)) * sz;
     float tx = (bx - px) / dx;
     float ty = (by - py) / dy;
     float tz = (bz - pz) / dz;
@@ -97,9 +99,9 @@ __device__ __forceinline__ float dda_step_r(float px, float py, float pz, float 
                                             float dz, float rx, float ry, float rz, float sx,
                                             float sy, float sz, float& qx, float& qy, float& qz) {
 #pragma clang fp contract(off)
-    float bx = floorf(px * sx + 1.0f) * sx;
-    float by = floorf(py * sy + 1.0f) * sy;
-    float bz = floorf(pz * sz + 1.0f) * sz;
+    float bx = floorf(__builtin_fmaf(px, sx, 1.0f)) * sx;
+    float by = floorf(__builtin_fmaf(py, sy, 1.0f)) * sy;
+    float bz = floorf(__builtin_fmaf(pz, sz, 1.0f)) * sz;
     float tx = div_recip(bx - px, dx, rx);
     float ty = div_recip(by - py, dy, ry);
     float tz = div_recip(bz - pz, dz, rz);
