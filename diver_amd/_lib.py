@@ -24,7 +24,7 @@ EXPORTS = [
     'diver_adam_step', 'diver_decoder_update_range', 'diver_ray_suffix',
     'diver_render_rays_sparsity', 'diver_server_clock', 'diver_server_create',
     'diver_server_destroy', 'diver_server_add_scene', 'diver_server_submit', 'diver_server_wait',
-    'diver_server_take_completed', 'diver_server_history',
+    'diver_server_take_completed', 'diver_server_history', 'diver_adam_step_rows',
 ]
 
 # status codes of include/diver.h
@@ -125,6 +125,8 @@ def _declare(L):
         'diver_integrate_table_backward': [i32, i64, vp, vp, vp, vp, vp, vp],
         'diver_adam_step': [vp, vp, vp, vp, i64, ctypes.c_double, ctypes.c_double, fl, fl, fl, fl,
                             i32, vp],
+        'diver_adam_step_rows': [vp, vp, vp, vp, i64, vp, i64, i32, ctypes.c_double,
+                                 ctypes.c_double, fl, fl, fl, fl, i32, vp],
     }
     sig.update({
         'diver_server_create': [ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)],

@@ -181,6 +181,19 @@ class _FusedRender(Function):
         return gv, gp, None, None, None, None, None, None, None, None, None
 
 
+def corner_rows(mask: torch.Tensor) -> torch.Tensor:
+    """int64 [K], sorted: the corners z (N+1)^2 + y (N+1) + x of the occupied voxels of an [N]^3
+    mask, i.e. the rows of the [(N+1)^3, C] grid that the integrals of occupied voxels read."""
+    m = mask.bool()
+    n = m.shape[0]
+    c = torch.zeros((n + 1,) * 3, dtype=torch.bool, device=m.device)
+    for dz in (0, 1):
+        for dy in (0, 1):
+            for dx in (0, 1):
+                c[dz:dz + n, dy:dy + n, dx:dx + n] |= m
+    return torch.nonzero(c.reshape(-1)).reshape(-1).contiguous()
+
+
 class BenchmarkDecoder(nn.Module):
     """Trainable benchmark decoder: density softplus(f[0]); colour MLP on [f[1:], PE(d)]
     (Linear (C-1+27)->H, ReLU, Linear H->H, ReLU, Linear H->3, sigmoid)."""
@@ -226,6 +239,11 @@ class FusedRenderer(nn.Module):
         self.handle = Decoder(canon)
         self.traversal = traversal
         self.accumulate_grad = accumulate_grad
+
+    def trainable_rows(self) -> torch.Tensor:
+        """The rows of ``voxels`` a walk can send a gradient to (``corner_rows`` of the mask, which
+        is fixed); the ``rows`` of an ``optim.Adam`` group for the grid."""
+        return corner_rows(self.grid.mask)
 
     def forward(self, o, d, t_stop: float = 1e-4, white_back: bool = True,
                 sparsity: bool = False):

@@ -462,6 +462,29 @@ int diver_adam_step(float* param, float* grad, float* exp_avg, float* exp_avg_sq
                       "adam_step");
 }
 
+int diver_adam_step_rows(float* param, float* grad, float* exp_avg, float* exp_avg_sq, int64_t n,
+                         const int64_t* rows, int64_t n_rows, int row_len, double beta1,
+                         double beta2, float eps, float weight_decay, float step_size,
+                         float bc2_sqrt, int clear_grad, diver_stream_t stream) {
+    REQUIRE(n >= 0 && n_rows >= 0, "negative size");
+    if (n_rows == 0) return DIVER_OK;
+    REQUIRE(param && grad && exp_avg && exp_avg_sq && rows, "null pointer");
+    REQUIRE(row_len > 0 && row_len % 4 == 0 && n % row_len == 0,
+            "row_len must be a positive multiple of 4 dividing n");
+    const uintptr_t al = reinterpret_cast<uintptr_t>(param) | reinterpret_cast<uintptr_t>(grad) |
+                         reinterpret_cast<uintptr_t>(exp_avg) |
+                         reinterpret_cast<uintptr_t>(exp_avg_sq);
+    REQUIRE(al % 16 == 0, "parameter, gradient and moment arrays must be 16-byte aligned");
+    REQUIRE(weight_decay == 0.0f,
+            "a row-restricted step equals the full one only without weight decay");
+    REQUIRE(beta1 >= 0.0 && beta1 < 1.0 && beta2 >= 0.0 && beta2 < 1.0, "betas outside [0, 1)");
+    REQUIRE(eps >= 0.0f && bc2_sqrt > 0.0f && std::isfinite(step_size), "bad Adam scalars");
+    return hip_status(diver::launch_adam_rows(param, grad, exp_avg, exp_avg_sq, n, rows, n_rows,
+                                              row_len, beta1, beta2, eps, step_size, bc2_sqrt,
+                                              clear_grad != 0, S(stream)),
+                      "adam_step_rows");
+}
+
 int diver_block_bits(const uint8_t* mask, int n, uint32_t* bits, diver_stream_t stream) {
     REQUIRE(mask && bits, "null pointer");
     REQUIRE(n >= 8 && n % 8 == 0, "n must be a positive multiple of 8");

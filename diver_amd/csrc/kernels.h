@@ -120,6 +120,9 @@ hipError_t launch_block_bits(const uint8_t* mask, int n, uint32_t* bits, hipStre
 hipError_t launch_adam(float* p, float* g, float* m, float* v, int64_t n, double b1, double b2,
                        float eps, float wd, float step_size, float bc2_sqrt, bool clear,
                        hipStream_t st);
+hipError_t launch_adam_rows(float* p, float* g, float* m, float* v, int64_t n, const int64_t* rows,
+                            int64_t n_rows, int row_len, double b1, double b2, float eps,
+                            float step_size, float bc2_sqrt, bool clear, hipStream_t st);
 hipError_t launch_brick_bits(const uint8_t* mask, int n, unsigned long long* bricks,
                              hipStream_t s);
 hipError_t launch_walk(const RenderArgs& a, int max_n, float* isect, int32_t* num, hipStream_t s);

@@ -106,6 +106,58 @@ __global__ void __launch_bounds__(256) adam_kernel(float* __restrict__ p, float*
     }
 }
 
+// The same update over listed rows only: row r covers elements [r * row_len, (r + 1) * row_len)
+// (row_len % 4 == 0, arrays 16-byte aligned), one float4 group per thread.  Rows outside
+// [0, n / row_len) are skipped.
+template <bool CLEAR>
+__global__ void __launch_bounds__(256) adam_rows_kernel(float* __restrict__ p, float* __restrict__ g,
+                                                        float* __restrict__ m, float* __restrict__ v,
+                                                        int64_t n_rows_total,
+                                                        const int64_t* __restrict__ rows,
+                                                        int64_t n_rows, int q, AdamScalars s) {
+    const int64_t units = n_rows * q;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    f4* P = reinterpret_cast<f4*>(p);
+    f4* G = reinterpret_cast<f4*>(g);
+    f4* M = reinterpret_cast<f4*>(m);
+    f4* V = reinterpret_cast<f4*>(v);
+    const f4 zero = {0.f, 0.f, 0.f, 0.f};
+    for (int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; u < units; u += stride) {
+        const int64_t k = u / q;
+        const int c = (int)(u - k * q);
+        const int64_t r = rows[k];
+        if (r < 0 || r >= n_rows_total) continue;
+        const int64_t i = r * q + c;
+        f4 pp = __builtin_nontemporal_load(P + i), gg = __builtin_nontemporal_load(G + i);
+        f4 mm = __builtin_nontemporal_load(M + i), vv = __builtin_nontemporal_load(V + i);
+        adam4(pp, gg, mm, vv, s);
+        __builtin_nontemporal_store(pp, P + i);
+        __builtin_nontemporal_store(mm, M + i);
+        __builtin_nontemporal_store(vv, V + i);
+        if (CLEAR) __builtin_nontemporal_store(zero, G + i);
+    }
+}
+
+hipError_t launch_adam_rows(float* p, float* g, float* m, float* v, int64_t n, const int64_t* rows,
+                            int64_t n_rows, int row_len, double b1, double b2, float eps,
+                            float step_size, float bc2_sqrt, bool clear, hipStream_t st) {
+    if (n_rows <= 0) return hipSuccess;
+    const AdamScalars s{(float)b1, (float)b2, (float)(1.0 - b1), (float)(1.0 - b2), eps, 0.0f,
+                        step_size, bc2_sqrt};
+    const int q = row_len / 4;
+    const int64_t units = n_rows * q;
+    int64_t blocks = (units + 255) / 256;
+    if (blocks > 256 * 32) blocks = 256 * 32;
+    const dim3 grid((unsigned)blocks), block(256);
+    if (clear)
+        hipLaunchKernelGGL((adam_rows_kernel<true>), grid, block, 0, st, p, g, m, v, n / row_len,
+                           rows, n_rows, q, s);
+    else
+        hipLaunchKernelGGL((adam_rows_kernel<false>), grid, block, 0, st, p, g, m, v, n / row_len,
+                           rows, n_rows, q, s);
+    return hipGetLastError();
+}
+
 hipError_t launch_adam(float* p, float* g, float* m, float* v, int64_t n, double b1, double b2,
                        float eps, float wd, float step_size, float bc2_sqrt, bool clear,
                        hipStream_t st) {

@@ -35,7 +35,7 @@ import torch
 
 from . import calls
 from . import synthetic as S
-from .adjoint import BenchmarkDecoder, FusedRenderer, positional_encoding
+from .adjoint import BenchmarkDecoder, FusedRenderer, corner_rows, positional_encoding
 from .decoder import Decoder, from_baseline
 from .model.diver import composite
 from .optim import Adam as DiverAdam
@@ -99,7 +99,13 @@ def run(path, steps, warmup, n_rays, dev, optimizer='diver'):
         params = [voxels] + list(dec.parameters())
         fwd = lambda o, d: stored_forward(voxels, dec, mask, 256, o, d)
     if optimizer == 'diver':
-        # one HIP pass per parameter that also clears its gradient (no zero_grad memset)
+        # one HIP pass per parameter that also clears its gradient (no zero_grad memset); the grid
+        # is updated on the corners of the occupied voxels only, the rows a gradient can reach
+        # (exactly the full update without weight decay)
+        rows = corner_rows(torch.from_numpy(sc.mask).to(dev))
+        opt = DiverAdam([{'params': [params[0]], 'rows': rows, 'row_len': 32},
+                         {'params': params[1:]}], lr=1e-3, clear_grad=True)
+    elif optimizer == 'diver-dense':
         opt = DiverAdam(params, lr=1e-3, clear_grad=True)
     else:
         # torch's fused update (its default multi-pass foreach Adam takes ~2.3x longer on the grid)
@@ -160,8 +166,10 @@ def main(argv=None):
     ap.add_argument('--warmup', type=int, default=3)
     ap.add_argument('--rays', type=int, default=8192)
     ap.add_argument('--path', choices=['both', 'recompute', 'stored'], default='both')
-    ap.add_argument('--optimizer', choices=['diver', 'torch'], default='diver',
-                    help="diver: one-pass HIP Adam (diver_adam_step); torch: torch's fused Adam")
+    ap.add_argument('--optimizer', choices=['diver', 'diver-dense', 'torch'], default='diver',
+                    help="diver: one-pass HIP Adam, the grid on its trainable rows "
+                         "(diver_adam_step_rows); diver-dense: the same over the whole grid; "
+                         "torch: torch's fused Adam")
     args = ap.parse_args(argv)
     dev = torch.device('cuda', 0)
     out = {'workload': 'training step, 256^3 x 32 trainable grid, hidden-64 benchmark decoder, '

@@ -324,6 +324,20 @@ int diver_adam_step(float* param, float* grad, float* exp_avg, float* exp_avg_sq
                     double beta1, double beta2, float eps, float weight_decay, float step_size,
                     float bc2_sqrt, int clear_grad, diver_stream_t stream);
 
+/* diver_adam_step over listed rows only: row r is elements [r * row_len, (r + 1) * row_len) of
+ * the n-element arrays (row_len a multiple of 4 dividing n, arrays 16-byte aligned; rows outside
+ * [0, n / row_len) are skipped).  Without weight decay (required: weight_decay == 0, the
+ * reference's default, configs/nerf_synthetic_fine.py 'weight_decay') an element whose gradient
+ * and both moments are zero is a fixed point of the update, so when every row that ever receives
+ * a gradient is listed, this step leaves the arrays exactly as diver_adam_step over all n
+ * elements would.  For the trainable grid those rows are the corners of the occupied voxels: the
+ * walks composite occupied voxels only, whose integrals read their 8 corners.  No counterpart in
+ * the reference (torch.optim.Adam updates the whole grid every step, train.py:39-47). */
+int diver_adam_step_rows(float* param, float* grad, float* exp_avg, float* exp_avg_sq, int64_t n,
+                         const int64_t* rows, int64_t n_rows, int row_len, double beta1,
+                         double beta2, float eps, float weight_decay, float step_size,
+                         float bc2_sqrt, int clear_grad, diver_stream_t stream);
+
 /* The occupied intervals the fused renderer composites, one walk per ray (diver_traversal):
  * the offline walk equals masked_intersect (utils/ray_voxel_intersection/src/
  * ray_voxel_intersect.cu:192-319); the real-time walk equals aabb_intersect followed by ray_march

@@ -89,3 +89,53 @@ def test_adam_unaligned_views(dev):
     torch.cuda.synchronize()
     assert pa.data_ptr() % 16 != 0 and pa.grad.data_ptr() % 16 != 0
     assert _close(pa, b)
+
+
+@pytest.mark.parametrize('clear', [False, True])
+def test_adam_rows_equals_full_step(dev, clear):
+    """The row-restricted step (diver_adam_step_rows) over every row that receives a gradient
+    leaves parameters and moments bit for bit as the full one-pass step over all elements, and
+    within rounding of torch.optim.Adam; rows never listed keep their values exactly."""
+    R, L = 3001, 32
+    g = torch.Generator(device='cpu').manual_seed(5)
+    p0 = torch.randn(R * L, generator=g)
+    rows = torch.randperm(R, generator=g)[:700].sort().values
+    touched = torch.zeros(R, dtype=torch.bool)
+    touched[rows] = True
+    grads = []
+    for _ in range(6):
+        gr = torch.randn(R, L, generator=g) * 1e-2
+        gr[~touched] = 0.0
+        grads.append(gr.reshape(-1).to(dev))
+    a = torch.nn.Parameter(p0.clone().to(dev))
+    b = torch.nn.Parameter(p0.clone().to(dev))
+    c = torch.nn.Parameter(p0.clone().to(dev))
+    ours = Adam([{'params': [a], 'rows': rows.to(dev), 'row_len': L}], lr=1e-2, clear_grad=clear)
+    full = Adam([b], lr=1e-2, clear_grad=clear)
+    ref = torch.optim.Adam([c], lr=1e-2)
+    for gr in grads:
+        a.grad = gr.clone()
+        b.grad = gr.clone()
+        c.grad = gr.clone()
+        ours.step()
+        full.step()
+        ref.step()
+        if clear:
+            assert float(a.grad.abs().max()) == 0.0
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
+    assert torch.equal(ours.state[a]['exp_avg'], full.state[b]['exp_avg'])
+    assert torch.equal(ours.state[a]['exp_avg_sq'], full.state[b]['exp_avg_sq'])
+    assert _close(a, c)
+    untouched = (~touched).repeat_interleave(L).to(dev)
+    assert torch.equal(a.detach()[untouched], p0.to(dev)[untouched])
+
+
+def test_adam_rows_refuses_weight_decay(dev):
+    from diver_amd import _lib
+    a = torch.nn.Parameter(torch.zeros(64, device=dev))
+    a.grad = torch.zeros_like(a)
+    opt = Adam([{'params': [a], 'rows': torch.arange(2, device=dev), 'row_len': 32}], lr=1e-2,
+               weight_decay=1e-2)
+    with pytest.raises(_lib.DiverError):
+        opt.step()
