@@ -78,21 +78,8 @@ def test_precision_report(dev):
         torch.cuda.synchronize()
         ms = ev[0].elapsed_time(ev[1]) / 10
         entry = {'ms_per_frame': round(ms, 4), 'rays_per_s': round(W * H / (ms / 1e3), 1)}
-        # matrix-core work per 64-row batch (two 32-column tiles; hidden and output layers padded
-        # to 32 rows), against the dense peak of the operand type
-        mt = HID // 32
-        if prec != 'fp32':
-            kc0, kch = C // 16, HID // 16
-            mfma_per_batch = 2 * (mt * kc0 + mt * kch + kch)
-            flops = np.ceil(intervals / 64) * mfma_per_batch * 32 * 32 * 16 * 2
-            entry['mfma_tflops'] = round(flops / (ms / 1e3) / 1e12, 2)
-            entry['mfma_util_vs_2500_tflops'] = round(flops / (ms / 1e3) / 2.5e15, 5)
-        else:
-            kh, kp = C // 2, HID // 2
-            mfma_per_batch = 2 * (mt * kh + mt * kp + kp)  # v_mfma_f32_32x32x2_f32
-            flops = np.ceil(intervals / 64) * mfma_per_batch * 32 * 32 * 2 * 2
-            entry['mfma_tflops'] = round(flops / (ms / 1e3) / 1e12, 2)
-            entry['mfma_util_vs_157_tflops'] = round(flops / (ms / 1e3) / 157.3e12, 5)
+        # the matrix-core rate of these renders is read from rocprofv3 counters
+        # (SQ_INSTS_VALU_MFMA_MOPS_*, SQ_VALU_MFMA_BUSY_CYCLES), not estimated here
         report[prec] = entry
     report['intervals_per_frame'] = intervals
     print(json.dumps(report))
