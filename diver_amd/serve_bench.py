@@ -6,16 +6,19 @@
 Scenes (synthetic, seeded): 128^3 x 32 channels, 256^3 x 32, 256^3 x 64, each with the benchmark
 decoder of its width.  Resolutions 200^2, 400^2, 800^2 and 1920x1080 and the scenes are drawn
 uniformly, cameras from the hemisphere set, arrival times from a Poisson process (seed 4).  The
-served configuration packs up to 16 pending requests of a scene into one launch with two launches
-in flight; ``--baseline`` serves the same arrivals one request per launch, one launch at a time,
+served configuration packs up to 16 pending requests of a scene into one launch with three
+launches in flight (``--streams``); ``--baseline`` serves the same arrivals one request per launch, one launch at a time,
 with the FP32 decoder.  Prints one JSON line: throughput (rays/s over the run), p50/p99 frame
 latency overall and per resolution, the unloaded latency of each resolution alone, launch counts.
 """
 from __future__ import annotations
 
 import argparse
+import collections
+import gc
 import json
 import sys
+import threading
 import time
 
 import numpy as np
@@ -44,7 +47,7 @@ def main(argv=None):
     ap.add_argument('--rate', type=float, default=400.0, help='mean frame arrivals per second')
     ap.add_argument('--seconds', type=float, default=4.0)
     ap.add_argument('--precision', default='bf16')
-    ap.add_argument('--streams', type=int, default=2)
+    ap.add_argument('--streams', type=int, default=3)
     ap.add_argument('--baseline', action='store_true',
                     help='one request per launch, one launch in flight, FP32 decoder')
     ap.add_argument('--split-rays', type=int, default=1 << 18,
@@ -72,10 +75,26 @@ def main(argv=None):
                 lat.append(r.latency * 1e3)
             alone[f'{w}x{h}'] = float(np.median(lat[1:]))
         print('[serve] arrivals', file=sys.stderr, flush=True)
+        # the client takes each frame as it is rendered and drops it (a collector thread), and
+        # keeps its start-up objects out of reach of Python's full garbage collections, whose
+        # pauses (tens of ms) would otherwise stop the submitting thread
+        q = collections.deque()
+        done = threading.Event()
+
+        def collect():
+            while not (done.is_set() and not q):
+                if not q:
+                    time.sleep(50e-6)
+                    continue
+                q.popleft().result(timeout=120)
+
+        gc.collect()
+        gc.freeze()
+        th = threading.Thread(target=collect)
+        th.start()
         t_start = time.perf_counter()
         probs = np.array([p for _r, p in RESOLUTIONS])
         t = t_start
-        reqs = []
         while True:
             t += rng.exponential(1.0 / args.rate)
             if t - t_start > args.seconds:
@@ -86,9 +105,9 @@ def main(argv=None):
             now = time.perf_counter()
             if t > now:
                 time.sleep(t - now)
-            reqs.append(srv.submit(name, cam, w, h))
-        for r in reqs:
-            r.result(timeout=120)
+            q.append(srv.submit(name, cam, w, h))
+        done.set()
+        th.join()
         st = srv.stats(since=t_start)
     st.update({'offered_rate_hz': args.rate, 'seconds': args.seconds, 'precision': args.precision,
                'streams': args.streams, 'max_requests_per_launch': max_requests,

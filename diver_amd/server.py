@@ -132,9 +132,16 @@ class RenderServer:
     1920x1080 frame, three 800x800 or sixteen 200x200 frames per launch.  Larger launches add
     latency without adding throughput once two launches are in flight (measured at 800x800:
     2^21 gives 2.79 G rays/s saturated and a 1.7 ms p99 at 2400 frames/s, 2^23 gives 2.62 G
-    rays/s and stalls the clients' output allocations)."""
+    rays/s and stalls the clients' output allocations).
 
-    def __init__(self, device=None, hbm_budget: Optional[int] = None, n_streams: int = 2,
+    ``n_streams`` defaults to 3.  HIP maps streams onto a few hardware queues (4 per process by
+    default) and shares a queue between streams once the process has more streams than queues;
+    kernels on one queue run one after another.  Once torch's stream pool exists (any
+    ``torch.cuda.Stream()``), two server streams were measured to share a queue: 800x800 frames
+    saturated at 2.15 G rays/s with two streams and 2.67 G with three (2.56 G and 2.70 G in a
+    process without the pool)."""
+
+    def __init__(self, device=None, hbm_budget: Optional[int] = None, n_streams: int = 3,
                  max_requests: int = _lib.MAX_REQUESTS, max_rays: int = 1 << 21,
                  split_rays: Optional[int] = 1 << 18):
         self.device = torch.device(device or 'cuda:0')
