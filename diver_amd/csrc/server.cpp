@@ -31,6 +31,7 @@ constexpr int64_t kBytesPerPixel = 20;  // rgba fp32 + int32 count
 constexpr size_t kHistory = 65536;
 constexpr size_t kKeepFinished = 1 << 20;
 constexpr double kRefresh = 50e-6;     // the retire thread re-reads the launch list this often
+constexpr size_t kSpareEvents = 4096;  // completed frames' ordering events kept for reuse
 
 double now_s() {
     timespec ts;
@@ -65,7 +66,7 @@ struct Frame {
     int64_t bytes;
     int64_t ray_ids;
     bool large;
-    hipEvent_t after;  // recorded on the caller's stream at submit, or null
+    hipEvent_t after;  // recorded on the caller's stream at submit
 };
 
 struct Launch {
@@ -87,6 +88,7 @@ struct diver_server {
     int64_t resident = 0;
     int64_t inflight_bytes = 0;
     std::deque<Frame*> pending;
+    std::vector<hipEvent_t> spare_events;                     // reused by submit
     std::vector<Launch> inflight;
     std::unordered_map<int64_t, Frame*> active;               // submitted, not complete
     std::unordered_map<int64_t, diver_frame_info> finished;   // complete, not waited for
@@ -107,6 +109,16 @@ struct diver_server {
 
 namespace {
 
+// Keeps a frame's ordering event for a later submit (the event has fired or was never waited on:
+// a launch waits on it before the frame completes).  Lock held.
+void release_event(diver_server* s, hipEvent_t e) {
+    if (!e) return;
+    if (s->spare_events.size() < kSpareEvents)
+        s->spare_events.push_back(e);
+    else
+        (void)hipEventDestroy(e);
+}
+
 // Moves a frame to the finished state.  Lock held.
 void finish_frame(diver_server* s, Frame* f, int status, double t) {
     f->info.status = status;
@@ -123,7 +135,7 @@ void finish_frame(diver_server* s, Frame* f, int status, double t) {
     s->history[s->hist_next] = f->info;
     s->hist_next = (s->hist_next + 1) % kHistory;
     s->hist_count = std::min(s->hist_count + 1, kHistory);
-    if (f->after) (void)hipEventDestroy(f->after);
+    release_event(s, f->after);
     delete f;
 }
 
@@ -203,7 +215,7 @@ void launch_loop(diver_server* s) {
             reqs[i].focal = f->focal;
             reqs[i].rgba = f->rgba;
             reqs[i].count = f->count;
-            if (f->after && he == hipSuccess) he = hipStreamWaitEvent(st, f->after, 0);
+            if (he == hipSuccess) he = hipStreamWaitEvent(st, f->after, 0);
         }
         int status = he == hipSuccess ? DIVER_OK : DIVER_EHIP;
         std::string msg = he == hipSuccess ? "" : hipGetErrorString(he);
@@ -341,6 +353,7 @@ int diver_server_destroy(diver_server* s) {
         if (kv.second->after) (void)hipEventDestroy(kv.second->after);
         delete kv.second;
     }
+    for (hipEvent_t e : s->spare_events) (void)hipEventDestroy(e);
     const bool failed = s->failed;
     const std::string msg = s->fail_msg;
     delete s;
@@ -389,14 +402,20 @@ int diver_server_submit(diver_server* s, int scene_id, const float* c2w_host, in
     f->bytes = (int64_t)width * height * kBytesPerPixel;
     f->ray_ids = (int64_t)((width + 7) / 8) * ((height + 7) / 8) * 64;
     f->after = nullptr;
-    if (after) {
-        hipError_t he = hipEventCreateWithFlags(&f->after, hipEventDisableTiming);
-        if (he == hipSuccess) he = hipEventRecord(f->after, reinterpret_cast<hipStream_t>(after));
-        if (he != hipSuccess) {
-            if (f->after) (void)hipEventDestroy(f->after);
-            delete f;
-            return err(DIVER_EHIP, "submit event: %s", hipGetErrorString(he));
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (!s->spare_events.empty()) {
+            f->after = s->spare_events.back();
+            s->spare_events.pop_back();
         }
+    }
+    hipError_t he = hipSuccess;
+    if (!f->after) he = hipEventCreateWithFlags(&f->after, hipEventDisableTiming);
+    if (he == hipSuccess) he = hipEventRecord(f->after, reinterpret_cast<hipStream_t>(after));
+    if (he != hipSuccess) {
+        if (f->after) (void)hipEventDestroy(f->after);
+        delete f;
+        return err(DIVER_EHIP, "submit event: %s", hipGetErrorString(he));
     }
     std::lock_guard<std::mutex> lk(s->mu);
     int code = DIVER_OK;
@@ -413,7 +432,7 @@ int diver_server_submit(diver_server* s, int scene_id, const float* c2w_host, in
         code = err(DIVER_EBUDGET, "%lld ray ids exceed the per-launch limit %lld",
                    (long long)f->ray_ids, (long long)s->cfg.max_rays);
     if (code != DIVER_OK) {
-        if (f->after) (void)hipEventDestroy(f->after);
+        release_event(s, f->after);
         delete f;
         return code;
     }

@@ -407,7 +407,7 @@ int diver_server_add_scene(diver_server* srv, const diver_grid* grid, const dive
 /* Queues one frame: c2w_host row-major 3x4 (copied), rgba [height][width][4] and count
  * [height][width] (optional) device buffers that the caller keeps alive until the frame is
  * complete.  Work the caller enqueued on `after` before this call (e.g. earlier use of the
- * output memory) is ordered before the frame's launch; NULL orders nothing.  Returns at once
+ * output memory) is ordered before the frame's launch; NULL is the null stream.  Returns at once
  * with *ticket. */
 int diver_server_submit(diver_server* srv, int scene_id, const float* c2w_host, int width,
                         int height, float focal, float* rgba, int32_t* count,
