@@ -16,11 +16,16 @@
 //      row's voxel, then the colour decoder on the matrix cores, one 32-row column tile at a
 //      time (BF16/FP16 operands, or FP32 operands with v_mfma_f32_32x32x2_f32), and an LDS
 //      atomic add of weight * colour into the owner ray's accumulator.  The per-ray view term
-//      W_v PE(d) + b_c is computed once per ray in FP32 and kept in LDS.
-// Empty-space skipping: the workgroup keeps the 8^3-block occupancy bitmap in LDS and each lane
-// caches the bit-packed mask of its current 4^3 brick; a voxel in an empty block never reads
-// memory.  Skipping changes no arithmetic of the walk, so the intervals are bit-identical to the
-// drop-in traversal kernels.
+//      W_v PE(d) + b_c is computed once per ray in FP32 and kept in LDS (as BF16 pairs for the
+//      BF16 decoder).
+// Empty-space skipping: each lane reads the 8^3-block occupancy bitmap (4 KB at 256^3, cached in
+// L1/L2) when it enters a new 4^3 brick and caches the bit-packed mask of that brick; a voxel in an
+// empty block never reads the mask.  Skipping changes no arithmetic of the walk, so the intervals
+// are bit-identical to the drop-in traversal kernels.
+// Occupancy: a wave needs 7.75 KB of LDS and 128 VGPRs with the BF16 decoder at hidden width 32, so
+// 16 waves fit a CU.  Frames in a batch finish 13 % sooner per frame than with the bitmap copied
+// to LDS and FP32 view terms (10 waves per CU); a lone frame takes ~6 % longer, its long central
+// tiles sharing their CUs with more waves.
 #include "common.h"
 #include "decoder.h"
 #include "kernels.h"
@@ -38,11 +43,22 @@ constexpr int kMaxLdsBlockWords = 8192;
 // several tiles per group a long tile would hold the LDS of its finished neighbours; single-wave
 // groups also let the dispatcher spread consecutive tiles over the XCDs one by one.
 constexpr int kWavesPerGroup = 1;
-constexpr int kBlocksInLds = 1;    // 8^3-block bitmap copied to LDS (1) or read through L1 (0)
+constexpr int kBlocksInLds = 0;    // 8^3-block bitmap copied to LDS (1) or read through L1 (0)
+// A wave still walking after this many phase-A iterations gets the highest issue priority: the
+// long central tiles finish sooner while short waves fill the gaps (2 % on batched frames).
+constexpr int kLongWalk = 256;
 constexpr int kGatherChunk = 4;    // channels per round of corner loads in the packed gather (4 or 8)
 constexpr int kSkip = 2;           // cheap steps per walk iteration over voxels known to be empty (0: off)
 
-__host__ __device__ constexpr int wave_lds_floats(int H) { return kQueue * kRowFloats + 64 * 3 + 64 * H; }
+// The per-ray view terms of the BF16 decoder are kept as packed BF16 pairs, the decoder's operand
+// precision (half the LDS of FP32 terms); the FP32 and FP16 decoders keep them in FP32.
+__host__ __device__ constexpr bool view_packed(int prec) { return prec == kPrecBF16; }
+__host__ __device__ constexpr int wave_lds_floats(int H, int prec) {
+    return kQueue * kRowFloats + 64 * 3 + (view_packed(prec) ? 32 * H : 64 * H);
+}
+
+__device__ __forceinline__ float bf16_lo(uint32_t w) { return __builtin_bit_cast(float, w << 16); }
+__device__ __forceinline__ float bf16_hi(uint32_t w) { return __builtin_bit_cast(float, w & 0xffff0000u); }
 
 // Segment integral of one row, packed for the matrix cores: zp[i] holds channels 2i and 2i+1
 // rounded to PREC.  Each channel is the FP32 sum over corners j = 0..7 (in that order) of
@@ -116,11 +132,20 @@ __device__ __forceinline__ void shade_tile(const RenderArgs& a, const float* que
     for (int mt = 0; mt < MT; ++mt) {
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
-            float4 h4 = *reinterpret_cast<const float4*>(hvs + owner * H + 32 * mt + 8 * g + (half >> 3));
-            y[mt][4 * g + 0] = h4.x;
-            y[mt][4 * g + 1] = h4.y;
-            y[mt][4 * g + 2] = h4.z;
-            y[mt][4 * g + 3] = h4.w;
+            if constexpr (view_packed(PREC)) {
+                const uint32_t* hvp = reinterpret_cast<const uint32_t*>(hvs);
+                uint2 h4 = *reinterpret_cast<const uint2*>(hvp + owner * (H / 2) + 16 * mt + 4 * g + (half >> 4));
+                y[mt][4 * g + 0] = bf16_lo(h4.x);
+                y[mt][4 * g + 1] = bf16_hi(h4.x);
+                y[mt][4 * g + 2] = bf16_lo(h4.y);
+                y[mt][4 * g + 3] = bf16_hi(h4.y);
+            } else {
+                float4 h4 = *reinterpret_cast<const float4*>(hvs + owner * H + 32 * mt + 8 * g + (half >> 3));
+                y[mt][4 * g + 0] = h4.x;
+                y[mt][4 * g + 1] = h4.y;
+                y[mt][4 * g + 2] = h4.z;
+                y[mt][4 * g + 3] = h4.w;
+            }
         }
 #pragma unroll
         for (int kc = 0; kc < KC0; ++kc) y[mt] = mfma32<PREC>(pk[(mt * KC0 + kc) * 64 + lane], b[kc], y[mt]);
@@ -292,8 +317,9 @@ __device__ __forceinline__ void skip_empty(RayWalk& ray, const OccCache& occ, in
     }
 }
 
-// Register budget: 3 waves per SIMD for hidden width 32.  Wider decoders keep 64 x H view terms
-// in LDS, which already limits a CU to 6 waves (1.5 per SIMD), so they take the 2-wave budget.
+// Register budget: 3 waves per SIMD for hidden width 32 (the BF16 kernel compiles to 128 VGPRs,
+// which allows 4).  Wider decoders keep 64 x H view terms in LDS (16 KB in FP32 at H = 64, 8 waves
+// per CU), so they take the 2-wave budget.
 template <class D>
 constexpr int tile_waves_per_simd() { return D::H > 32 ? 2 : 3; }
 
@@ -311,9 +337,9 @@ __global__ void __launch_bounds__(64 * kWavesPerGroup, tile_waves_per_simd<D>())
 
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
-    float* queue = wave_mem + wid * wave_lds_floats(H);
+    float* queue = wave_mem + wid * wave_lds_floats(H, PREC);
     float* acc = queue + kQueue * kRowFloats;
-    float* hvs = acc + 64 * 3;  // [64][H] per-ray view term
+    float* hvs = acc + 64 * 3;  // [64][H] per-ray view term (FP32, or [64][H / 2] packed pairs)
     acc[lane * 3 + 0] = 0.0f;
     acc[lane * 3 + 1] = 0.0f;
     acc[lane * 3 + 2] = 0.0f;
@@ -353,9 +379,17 @@ __global__ void __launch_bounds__(64 * kWavesPerGroup, tile_waves_per_simd<D>())
     if (live) {
         float hv[H];
         head_view<D>(a.W, dx, dy, dz, hv);
+        if constexpr (view_packed(PREC)) {
+            uint32_t* hvp = reinterpret_cast<uint32_t*>(hvs);
 #pragma unroll
-        for (int j = 0; j < H; j += 4)
-            *reinterpret_cast<float4*>(hvs + lane * H + j) = make_float4(hv[j], hv[j + 1], hv[j + 2], hv[j + 3]);
+            for (int j = 0; j < H; j += 4)
+                *reinterpret_cast<uint2*>(hvp + lane * (H / 2) + j / 2) =
+                    make_uint2(pack2<kPrecBF16>(hv[j], hv[j + 1]), pack2<kPrecBF16>(hv[j + 2], hv[j + 3]));
+        } else {
+#pragma unroll
+            for (int j = 0; j < H; j += 4)
+                *reinterpret_cast<float4*>(hvs + lane * H + j) = make_float4(hv[j], hv[j + 1], hv[j + 2], hv[j + 3]);
+        }
     }
     OccCache occ;
 
@@ -364,12 +398,14 @@ __global__ void __launch_bounds__(64 * kWavesPerGroup, tile_waves_per_simd<D>())
     int used = 0;
     float reg = 0.0f;  // sparsity term: sum of log(1 + s^2 / 0.5) over the evaluated intervals
     int qn = 0;  // queued rows (wave-uniform)
+    int iters = 0;  // phase-A iterations (wave-uniform)
     const float nf = (float)n;
 
     while (true) {
         // ---------------- phase A: walk, density, transmittance, enqueue ------------------------
         while (qn < kFill) {
             if (!wave_any(ray.alive)) break;
+            if (++iters == kLongWalk) __builtin_amdgcn_s_setprio(3);
             if constexpr (kSkip > 0) skip_empty<RT>(ray, occ, n);
             bool push = false;
             float px = 0.f, py = 0.f, pz = 0.f, qx = 0.f, qy = 0.f, qz = 0.f, wgt = 0.f;
@@ -543,10 +579,10 @@ hipError_t launch_pack_f32(int C, int H, int npost, const float* W, float* out, 
     return hipGetLastError();
 }
 
-static size_t render_tile_lds_bytes(int n, int H) {
+static size_t render_tile_lds_bytes(int n, int H, int prec) {
     int nb = n >> 3;
     int64_t nwords = kBlocksInLds ? ((int64_t)nb * nb * nb + 31) >> 5 : 0;
-    return (size_t)((nwords + 3) & ~3) * 4 + kWavesPerGroup * (size_t)wave_lds_floats(H) * sizeof(float);
+    return (size_t)((nwords + 3) & ~3) * 4 + kWavesPerGroup * (size_t)wave_lds_floats(H, prec) * sizeof(float);
 }
 
 bool tile_fits(int n) {
@@ -560,7 +596,7 @@ static hipError_t launch_t3(const BatchArgs& p, bool dch0, hipStream_t s) {
     constexpr int threads = 64 * kWavesPerGroup;
     const int64_t per_group = (int64_t)p.rays_per_wave * kWavesPerGroup;
     dim3 grid((unsigned)((p.total + per_group - 1) / per_group)), block(threads);
-    size_t lds = render_tile_lds_bytes(p.g.n, D::H);
+    size_t lds = render_tile_lds_bytes(p.g.n, D::H, PREC);
     if (dch0)
         hipLaunchKernelGGL((render_tile_kernel<D, PREC, RT, true>), grid, block, lds, s, p);
     else
