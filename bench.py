@@ -47,9 +47,15 @@ def counter_fractions(n, width, height, channels):
     if (n, width, height, channels) != (256, 800, 800, 32) or not os.path.exists(COUNTER_FILE):
         return {}
     with open(COUNTER_FILE) as f:
-        c = json.load(f)['bf16']
-    return {k: c[k] for k in ('limiter', 'valu_issue_frac', 'issue_frac', 'mfma_frac',
-                              'mfma_tflops', 'instructions_per_voxel_step') if k in c}
+        j = json.load(f)
+    c = j['bf16']
+    out = {k: c[k] for k in ('limiter', 'valu_issue_frac', 'issue_frac', 'mfma_frac',
+                             'mfma_tflops', 'instructions_per_voxel_step',
+                             'resident_waves_per_simd') if k in c}
+    cb = j.get('bf16_batch16', {})
+    out.update({'batch16_' + k: cb[k] for k in ('valu_issue_frac', 'issue_frac',
+                                                'resident_waves_per_simd') if k in cb})
+    return out
 
 
 def log(msg):
@@ -282,6 +288,17 @@ def main():
     torch.cuda.synchronize()
     two_stream_rays_s = R * args.steps / (time.perf_counter() - t0)
 
+    # ---- 16 frames in one launch (diver_render_batch, as the server packs frames): the steady
+    # throughput of the kernel without the frame-end tail
+    opts16 = calls.render_opts('bf16', 'realtime', t_stop, 0.0)
+    reqs16 = [{'c2w': np.asarray(cams[(7 * i) % len(cams)], np.float32), 'width': W, 'height': H,
+               'focal': focal_for(W, 0.69)} for i in range(16)]
+    outs16 = calls.render_batch(grid, dec, reqs16, opts16)
+    i_b16 = float(sum(int(cnt.sum()) for _rgba, cnt in outs16))  # intervals over the 16 frames
+    del outs16
+    lat_b16, _ = timed(lambda i: calls.render_batch(grid, dec, reqs16, opts16), 5)
+    batch16_rays_s = 16 * R / (float(np.median(lat_b16)) / 1e3)
+
     # ---- serving path: the native render server, frames of 800x800 -------------------------
     serve = None if args.no_serve else serving_legs(args, grid, dec, cams, W, H, t_stop)
     frame_rays_s = R * args.steps / wall
@@ -400,6 +417,7 @@ def main():
         'speedup_vs_plain_fp32': round(value / plain_rays_s, 2),
         'single_frame_speedup_vs_plain_fp32': round(frame_rays_s / plain_rays_s, 2),
         'two_frames_in_flight_rays_s': round(two_stream_rays_s, 1),
+        'batch16_rays_s': round(batch16_rays_s, 1),
         'intervals_per_frame': i_eval,
         # the render kernel, one 800x800 frame per launch: algorithmic bytes over its event-timed
         # launch duration, priced on the HBM roofline; 'limiter' is what bounds it instead
@@ -407,6 +425,10 @@ def main():
                      'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4),
                      'traffic': measured_traffic(args.n, W, H, C),
                      'kernel': 'render_tile_kernel', 'launch_ms': round(launch_ms, 4),
+                     # the same kernel over 16 frames per launch (the serving regime)
+                     'batch16_launch_ms': round(float(np.median(lat_b16)), 4),
+                     'batch16_achieved': round((16 * R * 40 + i_b16 * 8 * C * 4)
+                                               / (float(np.median(lat_b16)) / 1e3) / 1e9, 1),
                      **counter_fractions(args.n, W, H, C)},
         'cpu_baseline': cpu,
     }
