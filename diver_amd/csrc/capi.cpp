@@ -84,8 +84,10 @@ struct diver_decoder {
     bool density_ch0 = false;      // sigma = act(F[0]): w_s = e_0, b_s = 0, no pre-layers
     bool packs_valid = true;       // matrix-core packs match W (false after a device update)
     float* W = nullptr;            // canonical fp32
-    uint32_t* pack_bf16 = nullptr; // matrix-core A operands
+    uint32_t* pack_bf16 = nullptr; // matrix-core A operands (32x32x16 layout, round kernel)
     uint32_t* pack_f16 = nullptr;
+    uint32_t* pack16_bf16 = nullptr;  // matrix-core A operands (16x16x32 layout, tile kernel)
+    uint32_t* pack16_f16 = nullptr;
     float* pack_f32 = nullptr;     // FP32 matrix-core A operands (repacked on every update)
 };
 
@@ -257,6 +259,31 @@ static void pack_layer(const float* W, int M, int K, bool bf16, std::vector<uint
             }
 }
 
+// Matrix-core A operands of one layer W[M][K] for v_mfma_f32_16x16x32: per (16-row tile mt,
+// 32-wide k step kk, lane) the 8 k values of row 16*mt + lane%16 that lane group g = lane/16
+// supplies, in the order the tile kernel's B operands arrive in: 32*kk + 4*g + j for j < 4 and
+// 32*kk + 16 + 4*g + (j - 4) for j >= 4.  For the head layer these are the channels its
+// 4-lanes-per-row gather sums in a lane, for the hidden layers the units a lane holds in D tiles
+// 2*kk and 2*kk + 1.
+static void pack16_layer(const float* W, int M, int K, bool bf16, std::vector<uint32_t>& out) {
+    const int MT = (M + 15) / 16, KS = K / 32;
+    for (int mt = 0; mt < MT; ++mt)
+        for (int kk = 0; kk < KS; ++kk)
+            for (int lane = 0; lane < 64; ++lane) {
+                const int row = 16 * mt + (lane & 15), g = lane >> 4;
+                auto kof = [&](int j) {
+                    return j < 4 ? 32 * kk + 4 * g + j : 32 * kk + 16 + 4 * g + (j - 4);
+                };
+                for (int q = 0; q < 4; ++q) {
+                    float a = row < M ? W[(int64_t)row * K + kof(2 * q)] : 0.0f;
+                    float b = row < M ? W[(int64_t)row * K + kof(2 * q + 1)] : 0.0f;
+                    uint16_t ha = bf16 ? to_bf16(a) : to_f16(a);
+                    uint16_t hb = bf16 ? to_bf16(b) : to_f16(b);
+                    out.push_back((uint32_t)ha | ((uint32_t)hb << 16));
+                }
+            }
+}
+
 int diver_decoder_create(const diver_decoder_desc* desc, const float* params_host,
                          int64_t n_params, diver_decoder** out) {
     REQUIRE(desc && params_host && out, "null argument");
@@ -283,6 +310,8 @@ int diver_decoder_create(const diver_decoder_desc* desc, const float* params_hos
         if (d->W) (void)hipFree(d->W);
         if (d->pack_bf16) (void)hipFree(d->pack_bf16);
         if (d->pack_f16) (void)hipFree(d->pack_f16);
+        if (d->pack16_bf16) (void)hipFree(d->pack16_bf16);
+        if (d->pack16_f16) (void)hipFree(d->pack16_f16);
         if (d->pack_f32) (void)hipFree(d->pack_f32);
         delete d;
         return fail(code, "%s", what);
@@ -311,6 +340,20 @@ int diver_decoder_create(const diver_decoder_desc* desc, const float* params_hos
                 return cleanup(DIVER_ENOMEM, "decoder pack allocation failed");
             if (hipMemcpy(*dst, pk.data(), pk.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
                 return cleanup(DIVER_EHIP, "decoder pack upload failed");
+            if (C % 32 == 0 && H % 32 == 0) {
+                std::vector<uint32_t> p16;
+                pack16_layer(params_host + wc, H, C, prec == 0, p16);
+                for (int j = 0; j < desc->n_post; ++j)
+                    pack16_layer(params_host + post0 + (int64_t)j * (H * H + H), H, H, prec == 0,
+                                 p16);
+                pack16_layer(params_host + outo, 3, H, prec == 0, p16);
+                uint32_t** dst16 = prec == 0 ? &d->pack16_bf16 : &d->pack16_f16;
+                if (hipMalloc(dst16, p16.size() * 4) != hipSuccess)
+                    return cleanup(DIVER_ENOMEM, "decoder pack allocation failed");
+                if (hipMemcpy(*dst16, p16.data(), p16.size() * 4, hipMemcpyHostToDevice) !=
+                    hipSuccess)
+                    return cleanup(DIVER_EHIP, "decoder pack upload failed");
+            }
         }
         const int64_t nf = diver::pack_f32_size(C, H, desc->n_post);
         if (hipMalloc(&d->pack_f32, sizeof(float) * nf) != hipSuccess)
@@ -360,6 +403,8 @@ int diver_decoder_destroy(diver_decoder* d) {
     if (d->W) (void)hipFree(d->W);
     if (d->pack_bf16) (void)hipFree(d->pack_bf16);
     if (d->pack_f16) (void)hipFree(d->pack_f16);
+    if (d->pack16_bf16) (void)hipFree(d->pack16_bf16);
+    if (d->pack16_f16) (void)hipFree(d->pack16_f16);
     if (d->pack_f32) (void)hipFree(d->pack_f32);
     hipSetDevice(dev);
     delete d;
@@ -531,6 +576,9 @@ static int render_common(const diver_grid* g, const diver_decoder* dec,
     a.pack = precision == DIVER_BF16   ? dec->pack_bf16
              : precision == DIVER_FP16 ? dec->pack_f16
                                        : reinterpret_cast<const uint32_t*>(dec->pack_f32);
+    a.pack16 = precision == DIVER_BF16   ? dec->pack16_bf16
+               : precision == DIVER_FP16 ? dec->pack16_f16
+                                         : nullptr;
     a.t_stop = opts->t_stop;
     a.alpha_skip = opts->alpha_skip;
     return DIVER_OK;

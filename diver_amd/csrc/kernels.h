@@ -25,6 +25,7 @@ struct RenderArgs {
     float voxel_size;
     const float* W;        // canonical decoder params (fp32, device)
     const uint32_t* pack;  // matrix-core A operands for the chosen precision (device)
+    const uint32_t* pack16;  // BF16/FP16 A operands in the 16x16x32 layout (tile kernel)
     int64_t n_rays;        // explicit: number of rays; camera: tiles*64
     const float* o;
     const float* d;

@@ -60,6 +60,23 @@ __device__ __forceinline__ f32x16_t mfma32(u32x4_t a, u32x4_t b, f32x16_t c) {
     }
 }
 
+// v_mfma_f32_16x16x32_{bf16,f16} (gfx950), lane layouts (checked against a host matmul):
+//   A (16 x 32): lane l holds A[l % 16][8*(l/16) + 0..7]
+//   B (32 x 16): lane l holds B[8*(l/16) + 0..7][l % 16]
+//   D (16 x 16): lane l, register i holds D[4*(l/16) + i][l % 16]                (fp32)
+typedef __attribute__((ext_vector_type(4))) float f32x4_t;
+
+template <int PREC>
+__device__ __forceinline__ f32x4_t mfma16(u32x4_t a, u32x4_t b, f32x4_t c) {
+    if constexpr (PREC == kPrecBF16) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a),
+                                                       __builtin_bit_cast(bf16x8_t, b), c, 0, 0, 0);
+    } else {
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8_t, a),
+                                                      __builtin_bit_cast(f16x8_t, b), c, 0, 0, 0);
+    }
+}
+
 // FP32 products on the matrix cores: v_mfma_f32_32x32x2_f32 (FP32 inputs, FP32 accumulation).
 //   A (32 x 2): lane l holds A[l % 32][l / 32];   B (2 x 32): lane l holds B[l / 32][l % 32]
 //   D: the 32x32 layout above.

@@ -12,20 +12,19 @@
 //      softplus(f[0]) decoders), composites transmittance (compensated log-sum, early stop at
 //      T < t_stop) and pushes the row (entry, exit, weight = T*alpha, owner lane) into the
 //      wave's LDS queue.  The phase ends when >= 32 rows are queued or every lane is done.
-//   B. the wave takes up to 64 queued rows, one per lane: full FP32 segment integral of the
-//      row's voxel, then the colour decoder on the matrix cores, one 32-row column tile at a
-//      time (BF16/FP16 operands, or FP32 operands with v_mfma_f32_32x32x2_f32), and an LDS
-//      atomic add of weight * colour into the owner ray's accumulator.  The per-ray view term
-//      W_v PE(d) + b_c is computed once per ray in FP32 and kept in LDS (as BF16 pairs for the
-//      BF16 decoder).
+//   B. the wave takes up to 64 queued rows: full FP32 segment integral of each row's voxel, then
+//      the colour decoder on the matrix cores, and an LDS atomic add of weight * colour into the
+//      owner ray's accumulator.  BF16/FP16 decoders: 16-row tiles with v_mfma_f32_16x16x32, the
+//      integral gathered by 4 lanes per row so that each quad of lanes reads one 64-byte
+//      half-line per load (shade_batch16).  FP32 decoder: one row per lane, two 32-row column
+//      tiles with v_mfma_f32_32x32x2_f32.  The per-ray view term W_v PE(d) + b_c is computed
+//      once per ray in FP32 and kept in LDS (as BF16 pairs for the BF16 decoder).
 // Empty-space skipping: each lane reads the 8^3-block occupancy bitmap (4 KB at 256^3, cached in
 // L1/L2) when it enters a new 4^3 brick and caches the bit-packed mask of that brick; a voxel in an
 // empty block never reads the mask.  Skipping changes no arithmetic of the walk, so the intervals
 // are bit-identical to the drop-in traversal kernels.
-// Occupancy: a wave needs 7.75 KB of LDS and 128 VGPRs with the BF16 decoder at hidden width 32, so
-// 16 waves fit a CU.  Frames in a batch finish 13 % sooner per frame than with the bitmap copied
-// to LDS and FP32 view terms (10 waves per CU); a lone frame takes ~6 % longer, its long central
-// tiles sharing their CUs with more waves.
+// Occupancy: a wave needs 7.75 KB of LDS and at most 128 VGPRs with the BF16 decoder at hidden
+// width 32, so 16 waves fit a CU (10 with the bitmap copied to LDS and FP32 view terms).
 #include "common.h"
 #include "decoder.h"
 #include "kernels.h"
@@ -47,7 +46,7 @@ constexpr int kBlocksInLds = 0;    // 8^3-block bitmap copied to LDS (1) or read
 // A wave still walking after this many phase-A iterations gets the highest issue priority: the
 // long central tiles finish sooner while short waves fill the gaps (2 % on batched frames).
 constexpr int kLongWalk = 256;
-constexpr int kGatherChunk = 4;    // channels per round of corner loads in the packed gather (4 or 8)
+constexpr int kCornerRound = 4;   // corners whose feature loads are in flight together (16-bit phase B)
 constexpr int kSkip = 2;           // cheap steps per walk iteration over voxels known to be empty (0: off)
 
 // The per-ray view terms of the BF16 decoder are kept as packed BF16 pairs, the decoder's operand
@@ -60,128 +59,14 @@ __host__ __device__ constexpr int wave_lds_floats(int H, int prec) {
 __device__ __forceinline__ float bf16_lo(uint32_t w) { return __builtin_bit_cast(float, w << 16); }
 __device__ __forceinline__ float bf16_hi(uint32_t w) { return __builtin_bit_cast(float, w & 0xffff0000u); }
 
-// Segment integral of one row, packed for the matrix cores: zp[i] holds channels 2i and 2i+1
-// rounded to PREC.  Each channel is the FP32 sum over corners j = 0..7 (in that order) of
-// w[j] * voxels[off[j] + channel], exactly as gather_integral computes it; packing each chunk as
-// soon as its sums are done halves the registers the row occupies, which leaves room for the
-// loads of a chunk (8 corners x CH channels) to be in flight together.
-template <int C, int PREC, int CH>
-__device__ __forceinline__ void gather_packed(const float* __restrict__ voxels, const float (&w)[8],
-                                              const uint32_t (&off)[8], uint32_t (&zp)[C / 2]) {
-    static_assert(CH == 4 || CH == 8, "chunk of 4 or 8 channels");
-    static_assert(C % CH == 0, "channel count must be a multiple of the chunk");
-#pragma unroll
-    for (int c = 0; c < C / CH; ++c) {
-        float4 q[8][CH / 4];
-#pragma unroll
-        for (int j = 0; j < 8; ++j)
-#pragma unroll
-            for (int h = 0; h < CH / 4; ++h)
-                q[j][h] = reinterpret_cast<const float4*>(voxels + off[j])[c * (CH / 4) + h];
-#pragma unroll
-        for (int h = 0; h < CH / 4; ++h) {
-            float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                s0 += w[j] * q[j][h].x;
-                s1 += w[j] * q[j][h].y;
-                s2 += w[j] * q[j][h].z;
-                s3 += w[j] * q[j][h].w;
-            }
-            zp[(c * CH + 4 * h) / 2] = pack2<PREC>(s0, s1);
-            zp[(c * CH + 4 * h) / 2 + 1] = pack2<PREC>(s2, s3);
-        }
-        __builtin_amdgcn_sched_barrier(0);
-    }
-}
-
-// The packed row as the two B operands per 16-channel k chunk (rows_to_b's layout).
-template <int K>
-__device__ __forceinline__ void packed_to_b(const uint32_t (&zp)[K / 2], u32x4_t (&b0)[K / 16],
-                                            u32x4_t (&b1)[K / 16]) {
-#pragma unroll
-    for (int kc = 0; kc < K / 16; ++kc) {
-        uint32_t x[4], y[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            x[q] = zp[8 * kc + q];
-            y[q] = zp[8 * kc + 4 + q];
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) swap32(x[q], y[q]);
-        b0[kc] = u32x4_t{x[0], x[1], x[2], x[3]};
-        b1[kc] = u32x4_t{y[0], y[1], y[2], y[3]};
-    }
-}
-
-template <class D, int PREC>
-__device__ __forceinline__ void shade_tile(const RenderArgs& a, const float* queue,
-                                           const float* hvs, float* acc, int t, int nbatch,
-                                           const u32x4_t (&b)[D::Z / 16], int lane) {
-    constexpr int H = D::H, Z = D::Z;
-    constexpr int KC0 = Z / 16, MT = H / 32, KCH = H / 16;
-    const u32x4_t* pk = reinterpret_cast<const u32x4_t*>(a.pack);
-    const int c = lane & 31;
-    const int half = lane & 32;
-    const int row = 32 * t + c;
-    const bool live = row < nbatch;
-    const int owner = live ? __builtin_bit_cast(int, queue[row * kRowFloats + 7]) : 0;
-    // head: acc = W_c z + (W_v PE(d) + b_c) of the row's ray
-    f32x16_t y[MT];
-#pragma unroll
-    for (int mt = 0; mt < MT; ++mt) {
-#pragma unroll
-        for (int g = 0; g < 4; ++g) {
-            if constexpr (view_packed(PREC)) {
-                const uint32_t* hvp = reinterpret_cast<const uint32_t*>(hvs);
-                uint2 h4 = *reinterpret_cast<const uint2*>(hvp + owner * (H / 2) + 16 * mt + 4 * g + (half >> 4));
-                y[mt][4 * g + 0] = bf16_lo(h4.x);
-                y[mt][4 * g + 1] = bf16_hi(h4.x);
-                y[mt][4 * g + 2] = bf16_lo(h4.y);
-                y[mt][4 * g + 3] = bf16_hi(h4.y);
-            } else {
-                float4 h4 = *reinterpret_cast<const float4*>(hvs + owner * H + 32 * mt + 8 * g + (half >> 3));
-                y[mt][4 * g + 0] = h4.x;
-                y[mt][4 * g + 1] = h4.y;
-                y[mt][4 * g + 2] = h4.z;
-                y[mt][4 * g + 3] = h4.w;
-            }
-        }
-#pragma unroll
-        for (int kc = 0; kc < KC0; ++kc) y[mt] = mfma32<PREC>(pk[(mt * KC0 + kc) * 64 + lane], b[kc], y[mt]);
-    }
-    int pofs = MT * KC0;
-    static_assert(D::NPOST <= 1, "one post layer at most");
-    if constexpr (D::NPOST == 1) {
-        u32x4_t e[KCH];
-#pragma unroll
-        for (int mt = 0; mt < MT; ++mt) d_to_b_relu<PREC>(y[mt], e[2 * mt], e[2 * mt + 1]);
-#pragma unroll
-        for (int mt = 0; mt < MT; ++mt) {
-            f32x16_t s;
-#pragma unroll
-            for (int i = 0; i < 16; ++i)
-                s[i] = a.W[D::post_off(0) + H * H + 32 * mt + 8 * (i >> 2) + (half >> 3) + (i & 3)];
-#pragma unroll
-            for (int kc = 0; kc < KCH; ++kc) s = mfma32<PREC>(pk[(pofs + mt * KCH + kc) * 64 + lane], e[kc], s);
-            y[mt] = s;
-        }
-        pofs += MT * KCH;
-    }
-    u32x4_t e[KCH];
-#pragma unroll
-    for (int mt = 0; mt < MT; ++mt) d_to_b_relu<PREC>(y[mt], e[2 * mt], e[2 * mt + 1]);
-    f32x16_t o = {};
-#pragma unroll
-    for (int kc = 0; kc < KCH; ++kc) o = mfma32<PREC>(pk[(pofs + kc) * 64 + lane], e[kc], o);
-    // output rows 0..2 sit in registers 0..2 of lanes 0..31, column = row of the tile
-    if (lane < 32 && live) {
-        const float* bo = a.W + D::out_off + 3 * H;
-        const float wgt = queue[row * kRowFloats + 6];
-        atomicAdd(&acc[owner * 3 + 0], wgt * sigmoidf(o[0] + bo[0]));
-        atomicAdd(&acc[owner * 3 + 1], wgt * sigmoidf(o[1] + bo[1]));
-        atomicAdd(&acc[owner * 3 + 2], wgt * sigmoidf(o[2] + bo[2]));
-    }
+// relu of two 16x16 D tiles (units 4g + 0..3 of each) packed as one 16x16x32 B operand: k slots
+// 0..3 from the first tile, 4..7 from the second.
+template <int PREC>
+__device__ __forceinline__ u32x4_t relu_b16(const f32x4_t& d0, const f32x4_t& d1) {
+    return u32x4_t{pack2<PREC>(fmaxf(d0[0], 0.0f), fmaxf(d0[1], 0.0f)),
+                   pack2<PREC>(fmaxf(d0[2], 0.0f), fmaxf(d0[3], 0.0f)),
+                   pack2<PREC>(fmaxf(d1[0], 0.0f), fmaxf(d1[1], 0.0f)),
+                   pack2<PREC>(fmaxf(d1[2], 0.0f), fmaxf(d1[3], 0.0f))};
 }
 
 // acc += sum over k steps q < K of A(step q) B(bv[q]) with FP32 operands; the A operands (64
@@ -271,6 +156,133 @@ __device__ __forceinline__ void shade_tile_f32(const RenderArgs& a, const float*
         atomicAdd(&acc[owner * 3 + 0], wgt * sigmoidf(o[0] + bo[0]));
         atomicAdd(&acc[owner * 3 + 1], wgt * sigmoidf(o[1] + bo[1]));
         atomicAdd(&acc[owner * 3 + 2], wgt * sigmoidf(o[2] + bo[2]));
+    }
+}
+
+// Phase B of the BF16/FP16 decoders with v_mfma_f32_16x16x32: the batch's rows are shaded 16 at a
+// time (one tile t).  The gather runs with 4 lanes per row: lane l takes row 16t + l/4 and the
+// 16-byte piece p = l%4 of each 64-byte half of the corners' feature rows, i.e. channels
+// 32kk + 4p + (0..3) and 32kk + 16 + 4p + (0..3) of each 32-wide k step kk.  The 4 lanes of a
+// quad then read one 64-byte half-line per load instruction (one cache access per quad, against
+// four when each lane gathers a row of its own: the texture path is what limits batched frames).
+// One ds_bpermute per dword moves the sums to the MFMA layout (lane 16g + c: row 16t + c, lane
+// group g), and the head's A operands take their k values in the same channel order.  The hidden
+// layers keep their D tiles in registers: relu(D tiles 2kk, 2kk + 1) is the next layer's B
+// operand for k step kk, its A operands packed in that unit order.
+template <class D, int PREC>
+__device__ __forceinline__ void shade_batch16(const RenderArgs& a, const float* queue,
+                                              const float* hvs, float* acc, int nbatch, int lane,
+                                              int n) {
+    constexpr int C = D::C, H = D::H;
+    constexpr int KS0 = C / 32, MT = H / 16, KSH = H / 32;
+    static_assert(D::Z == C && C % 32 == 0 && H % 32 == 0, "16x16x32 tiles need C, H in 32s");
+    static_assert(D::NPOST <= 1, "one post layer at most");
+    constexpr int post_pk = MT * KS0;                  // A-operand steps of the head layer
+    constexpr int out_pk = post_pk + D::NPOST * MT * KSH;
+    const u32x4_t* pk = reinterpret_cast<const u32x4_t*>(a.pack16);
+    const int col = lane & 15, g = lane >> 4;          // MFMA layout
+    const int gr = lane >> 2, gp = lane & 3;           // gather layout
+    const int from = (4 * col + g) * 4;                // byte index of the gather lane of (col, g)
+    for (int t = 0; 16 * t < nbatch; ++t) {
+        const int grow = 16 * t + gr;
+        u32x4_t b[KS0];
+        if (grow < nbatch) {
+            const float4 r0 = *reinterpret_cast<const float4*>(queue + grow * kRowFloats);
+            const float4 r1 = *reinterpret_cast<const float4*>(queue + grow * kRowFloats + 4);
+            float w[8];
+            uint32_t off[8];
+            segment_corners<C, uint32_t>(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, n, w, off);
+#pragma unroll
+            for (int kk = 0; kk < KS0; ++kk) {
+                // each channel is the FP32 sum over corners j = 0..7 in that order; the corners'
+                // loads go in kCornerRound-corner rounds to bound the registers in flight
+                float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f, s4 = 0.f, s5 = 0.f, s6 = 0.f, s7 = 0.f;
+#pragma unroll
+                for (int j0 = 0; j0 < 8; j0 += kCornerRound) {
+                    float4 q0[kCornerRound], q1[kCornerRound];
+#pragma unroll
+                    for (int j = 0; j < kCornerRound; ++j) {
+                        const float* src = a.voxels + off[j0 + j] + 32 * kk + 4 * gp;
+                        q0[j] = *reinterpret_cast<const float4*>(src);
+                        q1[j] = *reinterpret_cast<const float4*>(src + 16);
+                    }
+#pragma unroll
+                    for (int j = 0; j < kCornerRound; ++j) {
+                        s0 += w[j0 + j] * q0[j].x;
+                        s1 += w[j0 + j] * q0[j].y;
+                        s2 += w[j0 + j] * q0[j].z;
+                        s3 += w[j0 + j] * q0[j].w;
+                        s4 += w[j0 + j] * q1[j].x;
+                        s5 += w[j0 + j] * q1[j].y;
+                        s6 += w[j0 + j] * q1[j].z;
+                        s7 += w[j0 + j] * q1[j].w;
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                b[kk] = u32x4_t{pack2<PREC>(s0, s1), pack2<PREC>(s2, s3), pack2<PREC>(s4, s5),
+                                pack2<PREC>(s6, s7)};
+            }
+        } else {
+#pragma unroll
+            for (int kk = 0; kk < KS0; ++kk) b[kk] = u32x4_t{0u, 0u, 0u, 0u};  // +0 in both halves
+        }
+#pragma unroll
+        for (int kk = 0; kk < KS0; ++kk)
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                b[kk][q] = (uint32_t)__builtin_amdgcn_ds_bpermute(from, (int)b[kk][q]);
+        const int row = 16 * t + col;
+        const bool live = row < nbatch;
+        int owner = 0;
+        float wgt = 0.0f;
+        if (live) {
+            const float2 ow = *reinterpret_cast<const float2*>(queue + row * kRowFloats + 6);
+            wgt = ow.x;
+            owner = __builtin_bit_cast(int, ow.y);
+        }
+        // head: W_c z + (W_v PE(d) + b_c) of the row's ray; lane holds units 16 mt + 4 g + i
+        f32x4_t y[MT];
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+            if constexpr (view_packed(PREC)) {
+                const uint32_t* hvp = reinterpret_cast<const uint32_t*>(hvs);
+                const uint2 h = *reinterpret_cast<const uint2*>(hvp + owner * (H / 2) + 8 * mt + 2 * g);
+                y[mt] = f32x4_t{bf16_lo(h.x), bf16_hi(h.x), bf16_lo(h.y), bf16_hi(h.y)};
+            } else {
+                const float4 h = *reinterpret_cast<const float4*>(hvs + owner * H + 16 * mt + 4 * g);
+                y[mt] = f32x4_t{h.x, h.y, h.z, h.w};
+            }
+#pragma unroll
+            for (int kk = 0; kk < KS0; ++kk)
+                y[mt] = mfma16<PREC>(pk[(mt * KS0 + kk) * 64 + lane], b[kk], y[mt]);
+        }
+        if constexpr (D::NPOST == 1) {
+            u32x4_t e[KSH];
+#pragma unroll
+            for (int kk = 0; kk < KSH; ++kk) e[kk] = relu_b16<PREC>(y[2 * kk], y[2 * kk + 1]);
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+                const float* bp = a.W + D::post_off(0) + H * H + 16 * mt + 4 * g;
+                f32x4_t s = f32x4_t{bp[0], bp[1], bp[2], bp[3]};
+#pragma unroll
+                for (int kk = 0; kk < KSH; ++kk)
+                    s = mfma16<PREC>(pk[(post_pk + mt * KSH + kk) * 64 + lane], e[kk], s);
+                y[mt] = s;
+            }
+        }
+        u32x4_t e[KSH];
+#pragma unroll
+        for (int kk = 0; kk < KSH; ++kk) e[kk] = relu_b16<PREC>(y[2 * kk], y[2 * kk + 1]);
+        f32x4_t o = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kk = 0; kk < KSH; ++kk) o = mfma16<PREC>(pk[(out_pk + kk) * 64 + lane], e[kk], o);
+        // outputs 0..2 of the row sit in registers 0..2 of lane group 0
+        if (g == 0 && live) {
+            const float* bo = a.W + D::out_off + 3 * H;
+            atomicAdd(&acc[owner * 3 + 0], wgt * sigmoidf(o[0] + bo[0]));
+            atomicAdd(&acc[owner * 3 + 1], wgt * sigmoidf(o[1] + bo[1]));
+            atomicAdd(&acc[owner * 3 + 2], wgt * sigmoidf(o[2] + bo[2]));
+        }
     }
 }
 
@@ -472,15 +484,15 @@ __global__ void __launch_bounds__(64 * kWavesPerGroup, tile_waves_per_simd<D>())
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         const int nbatch = qn < 64 ? qn : 64;
-        const bool mine = lane < nbatch;
-        float w[8];
-        uint32_t off[8];
-        if (mine) {
-            float4 r0 = *reinterpret_cast<const float4*>(queue + lane * kRowFloats);
-            float4 r1 = *reinterpret_cast<const float4*>(queue + lane * kRowFloats + 4);
-            segment_corners<C, uint32_t>(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, n, w, off);
-        }
         if constexpr (PREC == kPrecFP32) {
+            const bool mine = lane < nbatch;
+            float w[8];
+            uint32_t off[8];
+            if (mine) {
+                float4 r0 = *reinterpret_cast<const float4*>(queue + lane * kRowFloats);
+                float4 r1 = *reinterpret_cast<const float4*>(queue + lane * kRowFloats + 4);
+                segment_corners<C, uint32_t>(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, n, w, off);
+            }
             float z[Z];
             if (mine) {
                 gather_integral<C>(a.voxels, w, off, z);
@@ -493,18 +505,7 @@ __global__ void __launch_bounds__(64 * kWavesPerGroup, tile_waves_per_simd<D>())
             shade_tile_f32<D>(a, queue, hvs, acc, 0, nbatch, b0, lane);
             if (nbatch > 32) shade_tile_f32<D>(a, queue, hvs, acc, 1, nbatch, b1, lane);
         } else {
-            static_assert(Z == C, "the packed gather feeds the head input directly");
-            uint32_t zp[Z / 2];
-            if (mine) {
-                gather_packed<C, PREC, kGatherChunk>(a.voxels, w, off, zp);
-            } else {
-#pragma unroll
-                for (int i = 0; i < Z / 2; ++i) zp[i] = 0u;  // +0 in both halves
-            }
-            u32x4_t b0[Z / 16], b1[Z / 16];
-            packed_to_b<Z>(zp, b0, b1);
-            shade_tile<D, PREC>(a, queue, hvs, acc, 0, nbatch, b0, lane);
-            if (nbatch > 32) shade_tile<D, PREC>(a, queue, hvs, acc, 1, nbatch, b1, lane);
+            shade_batch16<D, PREC>(a, queue, hvs, acc, nbatch, lane, n);
         }
         // drop the shaded rows: move rows [64, qn) down to [0, qn - 64)
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -620,6 +621,7 @@ static hipError_t launch_t1(const BatchArgs& p, int prec, bool dch0, hipStream_t
 hipError_t launch_render_tiles(int variant, const BatchArgs& args, int prec, bool dch0,
                                hipStream_t s) {
     if (args.total == 0) return hipSuccess;
+    if (prec != kPrecFP32 && !args.g.pack16) return hipErrorInvalidValue;  // no 16x16x32 operands
     BatchArgs p = args;
     p.rays_per_wave = rays_per_wave(p.total);
     switch (variant) {
