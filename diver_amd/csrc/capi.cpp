@@ -463,7 +463,8 @@ int diver_walk_rays(const diver_grid* g, int traversal, int64_t n_rays, const fl
     REQUIRE(traversal == DIVER_WALK_OFFLINE || traversal == DIVER_WALK_REALTIME,
             "bad traversal %d", traversal);
     if (traversal == DIVER_WALK_REALTIME)
-        REQUIRE(g->octrees && g->n % 64 == 0, "the real-time walk needs the pyramid, n % 64 == 0");
+        REQUIRE(g->octrees && g->n % 64 == 0 && g->n >= 64 && g->n <= 4096,
+                "the real-time walk needs the pyramid and n a multiple of 64 in [64, 4096]");
     REQUIRE(g->blocks && g->bricks && g->n % 8 == 0,
             "the walk needs the block and brick bits (n % 8 == 0)");
     diver::RenderArgs a{};
@@ -549,7 +550,8 @@ static int render_common(const diver_grid* g, const diver_decoder* dec,
             "bad traversal %d", opts->traversal);
     if (opts->traversal == DIVER_WALK_REALTIME) {
         REQUIRE(g->octrees, "the real-time walk needs the occupancy pyramid");
-        REQUIRE(g->n >= 64 && g->n % 64 == 0, "the real-time walk needs n to be a multiple of 64");
+        REQUIRE(g->n >= 64 && g->n % 64 == 0 && g->n <= 4096,
+                "the real-time walk needs n to be a multiple of 64 in [64, 4096]");
     }
     if (dec->variant == 5)
         return fail(DIVER_EUNSUPPORTED,

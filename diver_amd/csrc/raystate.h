@@ -87,8 +87,10 @@ struct RayWalk {
         pz /= 64.0f;
         float qx = px, qy = py, qz = pz;
         int m = n / 64;
-        int64_t off = 0, total = 0;
-        for (int k = n / 64; k < n; k *= 4) total += (int64_t)k * k * k;
+        // cell indices fit 32 bits: the levels hold at most (n/4)^3 + (n/16)^3 + (n/64)^3 cells,
+        // below 2^31 for n <= 4096 (callers check n)
+        int off = 0, total = 0;
+        for (int k = n / 64; k < n; k *= 4) total += k * k * k;
         const int bound = max_dda_steps(nf) + 16;
         for (int it = 0; it < bound && m < n; ++it) {
             qx = px;
@@ -98,9 +100,9 @@ struct RayWalk {
             if (outside(px, py, pz, (float)m)) return;
             int vx, vy, vz;
             segment_voxel1(qx, qy, qz, px, py, pz, vx, vy, vz);
-            int64_t m3 = (int64_t)m * m * m;
-            int64_t ci = ((int64_t)vz * m + vy) * m + vx;
-            int64_t gi = off + ci;
+            const int m3 = m * m * m;
+            const int ci = (vz * m + vy) * m + vx;
+            const int gi = off + ci;
             const bool occ = (ci >= 0 && ci < m3 && gi < total) ? octrees[gi] != 0 : false;
             if (occ) {
                 off += m3;

@@ -293,7 +293,7 @@ __device__ __forceinline__ void shade_batch16(const RenderArgs& a, const float* 
 // test is not taken here; the full iteration takes it again (same operands, same result).  Up to
 // kSkip steps per call (more lets lanes that cannot skip wait longer); lanes stop one by one.
 template <bool RT>
-__device__ __forceinline__ void skip_empty(RayWalk& ray, const OccCache& occ, int n) {
+__device__ __forceinline__ void skip_empty(RayWalk& ray, const OccCache& occ, int n, bool all_fast) {
     const float nf = (float)n;
     bool sk = ray.alive && occ.brick >= 0;
 #pragma unroll 1
@@ -303,11 +303,9 @@ __device__ __forceinline__ void skip_empty(RayWalk& ray, const OccCache& occ, in
         float nx, ny, nz;
         dda_step_r(ray.lx, ray.ly, ray.lz, ray.dx, ray.dy, ray.dz, ray.rx, ray.ry, ray.rz, ray.sx,
                    ray.sy, ray.sz, nx, ny, nz);
-        if (wave_any(sk && !ray.fast)) {
-            if (!ray.fast)
-                dda_step(ray.lx, ray.ly, ray.lz, ray.dx, ray.dy, ray.dz, ray.sx, ray.sy, ray.sz, nx,
-                         ny, nz);
-        }
+        if (!all_fast && !ray.fast)
+            dda_step(ray.lx, ray.ly, ray.lz, ray.dx, ray.dy, ray.dz, ray.sx, ray.sy, ray.sz, nx, ny,
+                     nz);
         int vx, vy, vz;
         segment_voxel1(ray.lx, ray.ly, ray.lz, nx, ny, nz, vx, vy, vz);
         const bool inr = (unsigned)vx < (unsigned)n && (unsigned)vy < (unsigned)n &&
@@ -404,6 +402,8 @@ __global__ void __launch_bounds__(64 * kWavesPerGroup, tile_waves_per_simd<D>())
         }
     }
     OccCache occ;
+    // every live ray's direction takes the reciprocal-division step (wave-uniform)
+    const bool all_fast = !wave_any(ray.alive && !ray.fast);
 
     Kahan S;
     float T = 1.0f;
@@ -418,7 +418,7 @@ __global__ void __launch_bounds__(64 * kWavesPerGroup, tile_waves_per_simd<D>())
         while (qn < kFill) {
             if (!wave_any(ray.alive)) break;
             if (++iters == kLongWalk) __builtin_amdgcn_s_setprio(3);
-            if constexpr (kSkip > 0) skip_empty<RT>(ray, occ, n);
+            if constexpr (kSkip > 0) skip_empty<RT>(ray, occ, n, all_fast);
             bool push = false;
             float px = 0.f, py = 0.f, pz = 0.f, qx = 0.f, qy = 0.f, qz = 0.f, wgt = 0.f;
             if (ray.alive) {
@@ -427,7 +427,11 @@ __global__ void __launch_bounds__(64 * kWavesPerGroup, tile_waves_per_simd<D>())
                 } else {
                     ++ray.steps;
                     float nx, ny, nz;
-                    ray.step(ray.lx, ray.ly, ray.lz, nx, ny, nz);
+                    if (all_fast)
+                        dda_step_r(ray.lx, ray.ly, ray.lz, ray.dx, ray.dy, ray.dz, ray.rx, ray.ry,
+                                   ray.rz, ray.sx, ray.sy, ray.sz, nx, ny, nz);
+                    else
+                        ray.step(ray.lx, ray.ly, ray.lz, nx, ny, nz);
                     bool out = RT ? outside_open(nx, ny, nz, nf) : outside(nx, ny, nz, nf);
                     if (out) {
                         ray.alive = false;
