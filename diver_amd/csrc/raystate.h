@@ -158,35 +158,43 @@ struct RayWalk {
 struct OccCache {
     int brick = -1;
     uint32_t lo = 0, hi = 0;
-    int block = -1;            // 8^3 block of the cached brick
-    bool block_empty = false;  // that block has no occupied voxel
+    bool block_empty = false;  // the cached brick's 8^3 block has no occupied voxel
+    uint32_t shift = 2;        // the region a cheap step may stay in: the 8^3 block (3) or the brick (2)
+    uint32_t key = 0;          // that region's coordinates, 10 bits each: (z << 20) | (y << 10) | x
+
+    // occupancy of a voxel inside the grid, through the cached brick
+    __device__ __forceinline__ bool lookup_in(const uint32_t* blocks,
+                                              const unsigned long long* __restrict__ bricks, int n,
+                                              uint32_t ux, uint32_t uy, uint32_t uz) {
+        const uint32_t n4 = (uint32_t)n >> 2, n8 = (uint32_t)n >> 3;
+        const int br = (int)(__umul24(__umul24(uz >> 2, n4) + (uy >> 2), n4) + (ux >> 2));
+        if (br != brick) {
+            brick = br;
+            const uint32_t bid = __umul24(__umul24(uz >> 3, n8) + (uy >> 3), n8) + (ux >> 3);
+            // both words in flight together; the brick word of an empty block is zero
+            const uint32_t bw = blocks[bid >> 5];
+            const unsigned long long w = bricks[(uint32_t)br];
+            block_empty = !((bw >> (bid & 31u)) & 1u);
+            lo = (uint32_t)w;
+            hi = (uint32_t)(w >> 32);
+            shift = block_empty ? 3u : 2u;
+            key = ((uz >> shift) << 20) | ((uy >> shift) << 10) | (ux >> shift);
+        }
+        const uint32_t bit = ((uz & 3u) << 4) | ((uy & 3u) << 2) | (ux & 3u);
+        const uint32_t word = bit < 32u ? lo : hi;
+        return (word >> (bit & 31u)) & 1u;
+    }
 
     __device__ __forceinline__ bool lookup(const uint8_t* __restrict__ mask,
                                            const uint32_t* blocks,
                                            const unsigned long long* __restrict__ bricks, int n,
                                            int vx, int vy, int vz) {
-        if ((unsigned)vx < (unsigned)n && (unsigned)vy < (unsigned)n && (unsigned)vz < (unsigned)n) {
-            const int n4 = n >> 2;
-            int br = ((vz >> 2) * n4 + (vy >> 2)) * n4 + (vx >> 2);
-            if (br != brick) {
-                brick = br;
-                const int nb = n >> 3;
-                int bid = ((vz >> 3) * nb + (vy >> 3)) * nb + (vx >> 3);
-                block = bid;
-                block_empty = !((blocks[bid >> 5] >> (bid & 31)) & 1u);
-                if (!block_empty) {
-                    unsigned long long w = bricks[br];
-                    lo = (uint32_t)w;
-                    hi = (uint32_t)(w >> 32);
-                } else {
-                    lo = 0u;
-                    hi = 0u;
-                }
-            }
-            int bit = ((vz & 3) << 4) | ((vy & 3) << 2) | (vx & 3);
-            uint32_t word = bit < 32 ? lo : hi;
-            return (word >> (bit & 31)) & 1u;
-        }
+        const bool inr = (unsigned)vx < (unsigned)n && (unsigned)vy < (unsigned)n && (unsigned)vz < (unsigned)n;
+        // a voxel index outside the grid (the reference reads the byte mask at its flattened,
+        // wrapped index) is rare: the whole wave takes the general branch then
+        if (__builtin_expect(__builtin_amdgcn_ballot_w64(!inr) == 0, 1))
+            return lookup_in(blocks, bricks, n, (uint32_t)vx, (uint32_t)vy, (uint32_t)vz);
+        if (inr) return lookup_in(blocks, bricks, n, (uint32_t)vx, (uint32_t)vy, (uint32_t)vz);
         const int64_t ntot = (int64_t)n * n * n;
         int64_t ci = ((int64_t)vz * n + vy) * n + vx;
         return (ci >= 0 && ci < ntot) ? mask[ci] != 0 : false;
@@ -226,6 +234,13 @@ __device__ __forceinline__ bool next_cached(RayWalk& ray, OccCache& occ,
 // mlp_eval.cu:83-101) and the 8 corner offsets into a [(n+1)^3][C] grid.  I is the offset type:
 // uint32_t when the caller has checked (n+1)^3 * C < 2^32 (cheaper index arithmetic), else
 // int64_t.
+// The same with the cell (cx, cy, cz) = (min(int(p + 1e-4), int(q + 1e-4)) per axis) given: the
+// walk's segment voxel, int(min(p + 1e-4, q + 1e-4)), is that cell (int() is monotonic).
+template <int C, typename I = int64_t>
+__device__ __forceinline__ void segment_corners_cell(int cx, int cy, int cz, float px, float py,
+                                                     float pz, float qx, float qy, float qz, int n,
+                                                     float (&w)[8], I (&off)[8]);
+
 template <int C, typename I = int64_t>
 __device__ __forceinline__ void segment_corners(float px, float py, float pz, float qx, float qy,
                                                 float qz, int n, float (&w)[8], I (&off)[8]) {
@@ -239,6 +254,13 @@ __device__ __forceinline__ void segment_corners(float px, float py, float pz, fl
         cy = ay < by ? ay : by;
         cz = az < bz ? az : bz;
     }
+    segment_corners_cell<C, I>(cx, cy, cz, px, py, pz, qx, qy, qz, n, w, off);
+}
+
+template <int C, typename I>
+__device__ __forceinline__ void segment_corners_cell(int cx, int cy, int cz, float px, float py,
+                                                     float pz, float qx, float qy, float qz, int n,
+                                                     float (&w)[8], I (&off)[8]) {
     float x0 = px - cx, x1 = qx - cx, y0 = py - cy, y1 = qy - cy, z0 = pz - cz, z1 = qz - cz;
     float z01 = z0 + z1, y01 = y0 + y1, x01 = x0 + x1;
     w[7] = (2.0f * x0 * y0 * z0 + 2.0f * x1 * y1 * z1 + x01 * y01 * z01) * (1.0f / 12.0f);

@@ -1,546 +1,8 @@
-// Fused render kernel for decoders whose density comes from the integrated feature alone
-// (canonical form with no pre-layers): traversal and density run per lane, colour decoding runs
-// in dense 64-row matrix-core batches.
-//
-// One wavefront owns a tile of 64 consecutive ray ids of the launch (one ray per lane); the ids
-// run over a table of requests (camera views in 8x8 pixel tiles, or explicit ray batches), so
-// rays of several requests of different sizes share one launch.  A ray's result depends only on
-// its own intervals (its colour sum adds them front to back), never on the rays it shares a
-// wave or a launch with.  The wave alternates two phases:
-//   A. every live lane advances the exact DDA one voxel per iteration (lanes never wait on each
-//      other's hits); at an occupied voxel it integrates only what density needs (channel 0 for
-//      softplus(f[0]) decoders), composites transmittance (compensated log-sum, early stop at
-//      T < t_stop) and pushes the row (entry, exit, weight = T*alpha, owner lane) into the
-//      wave's LDS queue.  The phase ends when >= 32 rows are queued or every lane is done.
-//   B. the wave takes up to 64 queued rows: full FP32 segment integral of each row's voxel, then
-//      the colour decoder on the matrix cores, and an LDS atomic add of weight * colour into the
-//      owner ray's accumulator.  BF16/FP16 decoders: 16-row tiles with v_mfma_f32_16x16x32, the
-//      integral gathered by 4 lanes per row so that each quad of lanes reads one 64-byte
-//      half-line per load (shade_batch16).  FP32 decoder: one row per lane, two 32-row column
-//      tiles with v_mfma_f32_32x32x2_f32.  The per-ray view term W_v PE(d) + b_c is computed
-//      once per ray in FP32 and kept in LDS (as BF16 pairs for the BF16 decoder).
-// Empty-space skipping: each lane reads the 8^3-block occupancy bitmap (4 KB at 256^3, cached in
-// L1/L2) when it enters a new 4^3 brick and caches the bit-packed mask of that brick; a voxel in an
-// empty block never reads the mask.  Skipping changes no arithmetic of the walk, so the intervals
-// are bit-identical to the drop-in traversal kernels.
-// Occupancy: a wave needs 7.75 KB of LDS and at most 128 VGPRs with the BF16 decoder at hidden
-// width 32, so 16 waves fit a CU (10 with the bitmap copied to LDS and FP32 view terms).
-#include "common.h"
-#include "decoder.h"
-#include "kernels.h"
-#include "mfma.h"
-#include "raystate.h"
-#include "requests.h"
+// The fused render kernel's launches (render_tile.h holds the kernel): grids whose features fit
+// 32-bit byte offsets, the FP32 matrix-core operand packing and the launch dispatch.
+#include "render_tile.h"
 
 namespace diver {
-
-constexpr int kQueue = 96;       // rows per wave queue (phase A stops at >= 32, adds <= 64)
-constexpr int kFill = 32;        // phase A hands over to phase B at this many queued rows
-constexpr int kRowFloats = 8;    // p_in (3), p_out (3), weight, owner (as int bits)
-constexpr int kMaxLdsBlockWords = 8192;
-// One wave per workgroup: a workgroup's LDS is released only when its last wave ends, so with
-// several tiles per group a long tile would hold the LDS of its finished neighbours; single-wave
-// groups also let the dispatcher spread consecutive tiles over the XCDs one by one.
-constexpr int kWavesPerGroup = 1;
-constexpr int kBlocksInLds = 0;    // 8^3-block bitmap copied to LDS (1) or read through L1 (0)
-// A wave still walking after this many phase-A iterations gets the highest issue priority: the
-// long central tiles finish sooner while short waves fill the gaps (2 % on batched frames).
-constexpr int kLongWalk = 256;
-constexpr int kCornerRound = 4;   // corners whose feature loads are in flight together (16-bit phase B)
-constexpr int kSkip = 2;           // cheap steps per walk iteration over voxels known to be empty (0: off)
-
-// The per-ray view terms of the BF16 decoder are kept as packed BF16 pairs, the decoder's operand
-// precision (half the LDS of FP32 terms); the FP32 and FP16 decoders keep them in FP32.
-__host__ __device__ constexpr bool view_packed(int prec) { return prec == kPrecBF16; }
-__host__ __device__ constexpr int wave_lds_floats(int H, int prec) {
-    return kQueue * kRowFloats + 64 * 3 + (view_packed(prec) ? 32 * H : 64 * H);
-}
-
-__device__ __forceinline__ float bf16_lo(uint32_t w) { return __builtin_bit_cast(float, w << 16); }
-__device__ __forceinline__ float bf16_hi(uint32_t w) { return __builtin_bit_cast(float, w & 0xffff0000u); }
-
-// relu of two 16x16 D tiles (units 4g + 0..3 of each) packed as one 16x16x32 B operand: k slots
-// 0..3 from the first tile, 4..7 from the second.
-template <int PREC>
-__device__ __forceinline__ u32x4_t relu_b16(const f32x4_t& d0, const f32x4_t& d1) {
-    return u32x4_t{pack2<PREC>(fmaxf(d0[0], 0.0f), fmaxf(d0[1], 0.0f)),
-                   pack2<PREC>(fmaxf(d0[2], 0.0f), fmaxf(d0[3], 0.0f)),
-                   pack2<PREC>(fmaxf(d1[0], 0.0f), fmaxf(d1[1], 0.0f)),
-                   pack2<PREC>(fmaxf(d1[2], 0.0f), fmaxf(d1[3], 0.0f))};
-}
-
-// acc += sum over k steps q < K of A(step q) B(bv[q]) with FP32 operands; the A operands (64
-// floats per step at pk) are loaded 8 steps at a time, the next 8 in flight during the current
-// products, which bounds the registers they hold.
-template <int K>
-__device__ __forceinline__ f32x16_t mfma_chain_f32(const float* __restrict__ pk, int lane,
-                                                   const float (&bv)[K], f32x16_t acc) {
-    static_assert(K % 8 == 0, "k steps in chunks of 8");
-    float w[8];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) w[i] = pk[i * 64 + lane];
-#pragma unroll
-    for (int k0 = 0; k0 < K; k0 += 8) {
-        float wn[8];
-        if (k0 + 8 < K) {
-#pragma unroll
-            for (int i = 0; i < 8; ++i) wn[i] = pk[(k0 + 8 + i) * 64 + lane];
-        }
-#pragma unroll
-        for (int i = 0; i < 8; ++i) acc = mfma32_f32(w[i], bv[k0 + i], acc);
-        __builtin_amdgcn_sched_barrier(0);
-        if (k0 + 8 < K) {
-#pragma unroll
-            for (int i = 0; i < 8; ++i) w[i] = wn[i];
-        }
-    }
-    return acc;
-}
-
-// shade_tile with FP32 operands (v_mfma_f32_32x32x2_f32).  Packed A operands (floats, 64 per k
-// step): head W_c in natural k pairs (2kc, 2kc + 1) for (mt, kc); post and output layers in the
-// D order of d_order_feature for (mt, q), so relu(D register v of tile mt) is the B operand of
-// step 16 mt + v.  The output layer is padded to 32 rows.
-template <class D>
-__device__ __forceinline__ void shade_tile_f32(const RenderArgs& a, const float* queue,
-                                               const float* hvs, float* acc, int t, int nbatch,
-                                               const float (&b)[D::Z / 2], int lane) {
-    constexpr int H = D::H, Z = D::Z;
-    constexpr int MT = H / 32, KH = Z / 2, KP = H / 2;
-    const float* pk = reinterpret_cast<const float*>(a.pack);
-    // The operands are the same for every batch; an opaque pointer keeps their loads inside the
-    // batch (held across the walk they would take 128 registers at hidden width 64).
-    asm volatile("" : "+s"(pk));
-    const int c = lane & 31;
-    const int half = lane & 32;
-    const int row = 32 * t + c;
-    const bool live = row < nbatch;
-    const int owner = live ? __builtin_bit_cast(int, queue[row * kRowFloats + 7]) : 0;
-    f32x16_t y[MT];
-#pragma unroll
-    for (int mt = 0; mt < MT; ++mt) {
-#pragma unroll
-        for (int g = 0; g < 4; ++g) {
-            float4 h4 = *reinterpret_cast<const float4*>(hvs + owner * H + 32 * mt + 8 * g + (half >> 3));
-            y[mt][4 * g + 0] = h4.x;
-            y[mt][4 * g + 1] = h4.y;
-            y[mt][4 * g + 2] = h4.z;
-            y[mt][4 * g + 3] = h4.w;
-        }
-        y[mt] = mfma_chain_f32<KH>(pk + mt * KH * 64, lane, b, y[mt]);
-    }
-    int pofs = MT * KH * 64;
-    static_assert(D::NPOST <= 1, "one post layer at most");
-    if constexpr (D::NPOST == 1) {
-        float e[KP];
-#pragma unroll
-        for (int q = 0; q < KP; ++q) e[q] = fmaxf(y[q / 16][q % 16], 0.0f);
-#pragma unroll
-        for (int mt = 0; mt < MT; ++mt) {
-            f32x16_t s;
-#pragma unroll
-            for (int i = 0; i < 16; ++i)
-                s[i] = a.W[D::post_off(0) + H * H + 32 * mt + 8 * (i >> 2) + (half >> 3) + (i & 3)];
-            y[mt] = mfma_chain_f32<KP>(pk + pofs + mt * KP * 64, lane, e, s);
-        }
-        pofs += MT * KP * 64;
-    }
-    float e[KP];
-#pragma unroll
-    for (int q = 0; q < KP; ++q) e[q] = fmaxf(y[q / 16][q % 16], 0.0f);
-    f32x16_t o = {};
-    o = mfma_chain_f32<KP>(pk + pofs, lane, e, o);
-    if (lane < 32 && live) {
-        const float* bo = a.W + D::out_off + 3 * H;
-        const float wgt = queue[row * kRowFloats + 6];
-        atomicAdd(&acc[owner * 3 + 0], wgt * sigmoidf(o[0] + bo[0]));
-        atomicAdd(&acc[owner * 3 + 1], wgt * sigmoidf(o[1] + bo[1]));
-        atomicAdd(&acc[owner * 3 + 2], wgt * sigmoidf(o[2] + bo[2]));
-    }
-}
-
-// Phase B of the BF16/FP16 decoders with v_mfma_f32_16x16x32: the batch's rows are shaded 16 at a
-// time (one tile t).  The gather runs with 4 lanes per row: lane l takes row 16t + l/4 and the
-// 16-byte piece p = l%4 of each 64-byte half of the corners' feature rows, i.e. channels
-// 32kk + 4p + (0..3) and 32kk + 16 + 4p + (0..3) of each 32-wide k step kk.  The 4 lanes of a
-// quad then read one 64-byte half-line per load instruction (one cache access per quad, against
-// four when each lane gathers a row of its own: the texture path is what limits batched frames).
-// One ds_bpermute per dword moves the sums to the MFMA layout (lane 16g + c: row 16t + c, lane
-// group g), and the head's A operands take their k values in the same channel order.  The hidden
-// layers keep their D tiles in registers: relu(D tiles 2kk, 2kk + 1) is the next layer's B
-// operand for k step kk, its A operands packed in that unit order.
-template <class D, int PREC>
-__device__ __forceinline__ void shade_batch16(const RenderArgs& a, const float* queue,
-                                              const float* hvs, float* acc, int nbatch, int lane,
-                                              int n) {
-    constexpr int C = D::C, H = D::H;
-    constexpr int KS0 = C / 32, MT = H / 16, KSH = H / 32;
-    static_assert(D::Z == C && C % 32 == 0 && H % 32 == 0, "16x16x32 tiles need C, H in 32s");
-    static_assert(D::NPOST <= 1, "one post layer at most");
-    constexpr int post_pk = MT * KS0;                  // A-operand steps of the head layer
-    constexpr int out_pk = post_pk + D::NPOST * MT * KSH;
-    const u32x4_t* pk = reinterpret_cast<const u32x4_t*>(a.pack16);
-    const int col = lane & 15, g = lane >> 4;          // MFMA layout
-    const int gr = lane >> 2, gp = lane & 3;           // gather layout
-    const int from = (4 * col + g) * 4;                // byte index of the gather lane of (col, g)
-    for (int t = 0; 16 * t < nbatch; ++t) {
-        const int grow = 16 * t + gr;
-        u32x4_t b[KS0];
-        if (grow < nbatch) {
-            const float4 r0 = *reinterpret_cast<const float4*>(queue + grow * kRowFloats);
-            const float4 r1 = *reinterpret_cast<const float4*>(queue + grow * kRowFloats + 4);
-            float w[8];
-            uint32_t off[8];
-            segment_corners<C, uint32_t>(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, n, w, off);
-#pragma unroll
-            for (int kk = 0; kk < KS0; ++kk) {
-                // each channel is the FP32 sum over corners j = 0..7 in that order; the corners'
-                // loads go in kCornerRound-corner rounds to bound the registers in flight
-                float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f, s4 = 0.f, s5 = 0.f, s6 = 0.f, s7 = 0.f;
-#pragma unroll
-                for (int j0 = 0; j0 < 8; j0 += kCornerRound) {
-                    float4 q0[kCornerRound], q1[kCornerRound];
-#pragma unroll
-                    for (int j = 0; j < kCornerRound; ++j) {
-                        const float* src = a.voxels + off[j0 + j] + 32 * kk + 4 * gp;
-                        q0[j] = *reinterpret_cast<const float4*>(src);
-                        q1[j] = *reinterpret_cast<const float4*>(src + 16);
-                    }
-#pragma unroll
-                    for (int j = 0; j < kCornerRound; ++j) {
-                        s0 += w[j0 + j] * q0[j].x;
-                        s1 += w[j0 + j] * q0[j].y;
-                        s2 += w[j0 + j] * q0[j].z;
-                        s3 += w[j0 + j] * q0[j].w;
-                        s4 += w[j0 + j] * q1[j].x;
-                        s5 += w[j0 + j] * q1[j].y;
-                        s6 += w[j0 + j] * q1[j].z;
-                        s7 += w[j0 + j] * q1[j].w;
-                    }
-                    __builtin_amdgcn_sched_barrier(0);
-                }
-                b[kk] = u32x4_t{pack2<PREC>(s0, s1), pack2<PREC>(s2, s3), pack2<PREC>(s4, s5),
-                                pack2<PREC>(s6, s7)};
-            }
-        } else {
-#pragma unroll
-            for (int kk = 0; kk < KS0; ++kk) b[kk] = u32x4_t{0u, 0u, 0u, 0u};  // +0 in both halves
-        }
-#pragma unroll
-        for (int kk = 0; kk < KS0; ++kk)
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                b[kk][q] = (uint32_t)__builtin_amdgcn_ds_bpermute(from, (int)b[kk][q]);
-        const int row = 16 * t + col;
-        const bool live = row < nbatch;
-        int owner = 0;
-        float wgt = 0.0f;
-        if (live) {
-            const float2 ow = *reinterpret_cast<const float2*>(queue + row * kRowFloats + 6);
-            wgt = ow.x;
-            owner = __builtin_bit_cast(int, ow.y);
-        }
-        // head: W_c z + (W_v PE(d) + b_c) of the row's ray; lane holds units 16 mt + 4 g + i
-        f32x4_t y[MT];
-#pragma unroll
-        for (int mt = 0; mt < MT; ++mt) {
-            if constexpr (view_packed(PREC)) {
-                const uint32_t* hvp = reinterpret_cast<const uint32_t*>(hvs);
-                const uint2 h = *reinterpret_cast<const uint2*>(hvp + owner * (H / 2) + 8 * mt + 2 * g);
-                y[mt] = f32x4_t{bf16_lo(h.x), bf16_hi(h.x), bf16_lo(h.y), bf16_hi(h.y)};
-            } else {
-                const float4 h = *reinterpret_cast<const float4*>(hvs + owner * H + 16 * mt + 4 * g);
-                y[mt] = f32x4_t{h.x, h.y, h.z, h.w};
-            }
-#pragma unroll
-            for (int kk = 0; kk < KS0; ++kk)
-                y[mt] = mfma16<PREC>(pk[(mt * KS0 + kk) * 64 + lane], b[kk], y[mt]);
-        }
-        if constexpr (D::NPOST == 1) {
-            u32x4_t e[KSH];
-#pragma unroll
-            for (int kk = 0; kk < KSH; ++kk) e[kk] = relu_b16<PREC>(y[2 * kk], y[2 * kk + 1]);
-#pragma unroll
-            for (int mt = 0; mt < MT; ++mt) {
-                const float* bp = a.W + D::post_off(0) + H * H + 16 * mt + 4 * g;
-                f32x4_t s = f32x4_t{bp[0], bp[1], bp[2], bp[3]};
-#pragma unroll
-                for (int kk = 0; kk < KSH; ++kk)
-                    s = mfma16<PREC>(pk[(post_pk + mt * KSH + kk) * 64 + lane], e[kk], s);
-                y[mt] = s;
-            }
-        }
-        u32x4_t e[KSH];
-#pragma unroll
-        for (int kk = 0; kk < KSH; ++kk) e[kk] = relu_b16<PREC>(y[2 * kk], y[2 * kk + 1]);
-        f32x4_t o = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int kk = 0; kk < KSH; ++kk) o = mfma16<PREC>(pk[(out_pk + kk) * 64 + lane], e[kk], o);
-        // outputs 0..2 of the row sit in registers 0..2 of lane group 0
-        if (g == 0 && live) {
-            const float* bo = a.W + D::out_off + 3 * H;
-            atomicAdd(&acc[owner * 3 + 0], wgt * sigmoidf(o[0] + bo[0]));
-            atomicAdd(&acc[owner * 3 + 1], wgt * sigmoidf(o[1] + bo[1]));
-            atomicAdd(&acc[owner * 3 + 2], wgt * sigmoidf(o[2] + bo[2]));
-        }
-    }
-}
-
-// Cheap steps over voxels already known to be empty.  Before each full walk iteration a lane
-// takes exact DDA steps for as long as the new voxel lies inside the grid, in its cached 4^3
-// brick (or in the cached brick's 8^3 block when that block is empty) and has its occupancy bit
-// clear: the full iteration would only have stepped over such a voxel.  A step that fails the
-// test is not taken here; the full iteration takes it again (same operands, same result).  Up to
-// kSkip steps per call (more lets lanes that cannot skip wait longer); lanes stop one by one.
-template <bool RT>
-__device__ __forceinline__ void skip_empty(RayWalk& ray, const OccCache& occ, int n, bool all_fast) {
-    const float nf = (float)n;
-    bool sk = ray.alive && occ.brick >= 0;
-#pragma unroll 1
-    for (int k = 0; k < kSkip; ++k) {
-        sk = sk && ray.steps < ray.max_steps;
-        if (!wave_any(sk)) break;
-        float nx, ny, nz;
-        dda_step_r(ray.lx, ray.ly, ray.lz, ray.dx, ray.dy, ray.dz, ray.rx, ray.ry, ray.rz, ray.sx,
-                   ray.sy, ray.sz, nx, ny, nz);
-        if (!all_fast && !ray.fast)
-            dda_step(ray.lx, ray.ly, ray.lz, ray.dx, ray.dy, ray.dz, ray.sx, ray.sy, ray.sz, nx, ny,
-                     nz);
-        int vx, vy, vz;
-        segment_voxel1(ray.lx, ray.ly, ray.lz, nx, ny, nz, vx, vy, vz);
-        const bool inr = (unsigned)vx < (unsigned)n && (unsigned)vy < (unsigned)n &&
-                         (unsigned)vz < (unsigned)n;
-        const uint32_t n4 = (uint32_t)n >> 2, n8 = (uint32_t)n >> 3;
-        const uint32_t ux = (uint32_t)vx, uy = (uint32_t)vy, uz = (uint32_t)vz;
-        const uint32_t region = occ.block_empty
-                                    ? __umul24(__umul24(uz >> 3, n8) + (uy >> 3), n8) + (ux >> 3)
-                                    : __umul24(__umul24(uz >> 2, n4) + (uy >> 2), n4) + (ux >> 2);
-        const int cached = occ.block_empty ? occ.block : occ.brick;
-        const uint32_t bit = ((uz & 3u) << 4) | ((uy & 3u) << 2) | (ux & 3u);
-        const bool empty = occ.block_empty || !(((bit < 32u ? occ.lo : occ.hi) >> (bit & 31u)) & 1u);
-        sk = sk && inr && (int)region == cached && empty &&
-             !(RT ? outside_open(nx, ny, nz, nf) : outside(nx, ny, nz, nf));
-        ray.lx = sk ? nx : ray.lx;
-        ray.ly = sk ? ny : ray.ly;
-        ray.lz = sk ? nz : ray.lz;
-        ray.steps += sk ? 1 : 0;
-    }
-}
-
-// Register budget: 3 waves per SIMD for hidden width 32 (the BF16 kernel compiles to 128 VGPRs,
-// which allows 4).  Wider decoders keep 64 x H view terms in LDS (16 KB in FP32 at H = 64, 8 waves
-// per CU), so they take the 2-wave budget.
-template <class D>
-constexpr int tile_waves_per_simd() { return D::H > 32 ? 2 : 3; }
-
-template <class D, int PREC, bool RT, bool DCH0>
-__global__ void __launch_bounds__(64 * kWavesPerGroup, tile_waves_per_simd<D>()) render_tile_kernel(BatchArgs p) {
-    static_assert(D::NPRE == 0, "the tile kernel needs density from the integrated feature");
-    constexpr int C = D::C, H = D::H, Z = D::Z;
-    const RenderArgs& a = p.g;
-    extern __shared__ uint32_t smem[];
-    const int n = a.n;
-    const int nb = n >> 3;
-    const int nwords = kBlocksInLds ? (nb * nb * nb + 31) >> 5 : 0;
-    const uint32_t* blocks = kBlocksInLds ? smem : a.blocks;
-    float* wave_mem = reinterpret_cast<float*>(smem + ((nwords + 3) & ~3));
-
-    const int lane = threadIdx.x & 63;
-    const int wid = threadIdx.x >> 6;
-    float* queue = wave_mem + wid * wave_lds_floats(H, PREC);
-    float* acc = queue + kQueue * kRowFloats;
-    float* hvs = acc + 64 * 3;  // [64][H] per-ray view term (FP32, or [64][H / 2] packed pairs)
-    acc[lane * 3 + 0] = 0.0f;
-    acc[lane * 3 + 1] = 0.0f;
-    acc[lane * 3 + 2] = 0.0f;
-
-    // lanes [0, rays_per_wave) of each wave take consecutive ray ids; the others stay idle
-    const int rpw = p.rays_per_wave;
-    const int64_t g = ((int64_t)blockIdx.x * kWavesPerGroup + wid) * rpw + lane;
-    int req = 0;
-    int64_t out_idx = 0;
-    float ox = 0.f, oy = 0.f, oz = 0.f, dx = 0.f, dy = 0.f, dz = 1.f;
-    const bool valid = lane < rpw && g < p.total &&
-                       ray_of(p, g, req, out_idx, ox, oy, oz, dx, dy, dz);
-    if (!valid) {
-        ox = oy = oz = 0.f;
-        dx = dy = 0.f;
-        dz = 1.f;
-    }
-    RayWalk ray;
-    if constexpr (RT)
-        ray.init_rt(ox, oy, oz, dx, dy, dz, a.xyzmin, a.voxel_size, n, a.octrees, valid);
-    else
-        ray.init(ox, oy, oz, dx, dy, dz, a.xyzmin, a.voxel_size, (float)n, valid);
-
-    // Most tiles of a view miss the occupied cells entirely: such a wave skips the bitmap copy and
-    // the view term (nothing would read them) and goes straight to writing T = 1.
-    const bool live = kWavesPerGroup > 1 || wave_any(ray.alive);
-    if (kBlocksInLds && live) {
-        if ((nwords & 3) == 0 && (reinterpret_cast<uintptr_t>(a.blocks) & 15) == 0) {
-            const uint4* src = reinterpret_cast<const uint4*>(a.blocks);
-            uint4* dst = reinterpret_cast<uint4*>(smem);
-            for (int i = threadIdx.x; i < (nwords >> 2); i += blockDim.x) dst[i] = src[i];
-        } else {
-            for (int i = threadIdx.x; i < nwords; i += blockDim.x) smem[i] = a.blocks[i];
-        }
-        __syncthreads();
-    }
-    if (live) {
-        float hv[H];
-        head_view<D>(a.W, dx, dy, dz, hv);
-        if constexpr (view_packed(PREC)) {
-            uint32_t* hvp = reinterpret_cast<uint32_t*>(hvs);
-#pragma unroll
-            for (int j = 0; j < H; j += 4)
-                *reinterpret_cast<uint2*>(hvp + lane * (H / 2) + j / 2) =
-                    make_uint2(pack2<kPrecBF16>(hv[j], hv[j + 1]), pack2<kPrecBF16>(hv[j + 2], hv[j + 3]));
-        } else {
-#pragma unroll
-            for (int j = 0; j < H; j += 4)
-                *reinterpret_cast<float4*>(hvs + lane * H + j) = make_float4(hv[j], hv[j + 1], hv[j + 2], hv[j + 3]);
-        }
-    }
-    OccCache occ;
-    // every live ray's direction takes the reciprocal-division step (wave-uniform)
-    const bool all_fast = !wave_any(ray.alive && !ray.fast);
-
-    Kahan S;
-    float T = 1.0f;
-    int used = 0;
-    float reg = 0.0f;  // sparsity term: sum of log(1 + s^2 / 0.5) over the evaluated intervals
-    int qn = 0;  // queued rows (wave-uniform)
-    int iters = 0;  // phase-A iterations (wave-uniform)
-    const float nf = (float)n;
-
-    while (true) {
-        // ---------------- phase A: walk, density, transmittance, enqueue ------------------------
-        while (qn < kFill) {
-            if (!wave_any(ray.alive)) break;
-            if (++iters == kLongWalk) __builtin_amdgcn_s_setprio(3);
-            if constexpr (kSkip > 0) skip_empty<RT>(ray, occ, n, all_fast);
-            bool push = false;
-            float px = 0.f, py = 0.f, pz = 0.f, qx = 0.f, qy = 0.f, qz = 0.f, wgt = 0.f;
-            if (ray.alive) {
-                if (ray.steps >= ray.max_steps) {
-                    ray.alive = false;
-                } else {
-                    ++ray.steps;
-                    float nx, ny, nz;
-                    if (all_fast)
-                        dda_step_r(ray.lx, ray.ly, ray.lz, ray.dx, ray.dy, ray.dz, ray.rx, ray.ry,
-                                   ray.rz, ray.sx, ray.sy, ray.sz, nx, ny, nz);
-                    else
-                        ray.step(ray.lx, ray.ly, ray.lz, nx, ny, nz);
-                    bool out = RT ? outside_open(nx, ny, nz, nf) : outside(nx, ny, nz, nf);
-                    if (out) {
-                        ray.alive = false;
-                    } else {
-                        int vx, vy, vz;
-                        segment_voxel1(ray.lx, ray.ly, ray.lz, nx, ny, nz, vx, vy, vz);
-                        px = ray.lx; py = ray.ly; pz = ray.lz;
-                        ray.lx = nx; ray.ly = ny; ray.lz = nz;
-                        if (occ.lookup(a.mask, blocks, a.bricks, n, vx, vy, vz)) {
-                            qx = nx; qy = ny; qz = nz;
-                            float w[8];
-                            uint32_t off[8];
-                            segment_corners<C, uint32_t>(px, py, pz, qx, qy, qz, n, w, off);
-                            float s;
-                            if constexpr (DCH0) {
-                                s = 0.0f;
-#pragma unroll
-                                for (int j = 0; j < 8; ++j) s += w[j] * a.voxels[off[j]];
-                            } else {
-                                float f[C];
-                                gather_integral<C>(a.voxels, w, off, f);
-                                s = a.W[D::bs_off];
-#pragma unroll
-                                for (int i = 0; i < C; ++i) s += a.W[D::ws_off + i] * f[i];
-                            }
-                            float sigma, alpha;
-                            density_alpha<D>(s, sigma, alpha);
-                            used += 1;
-                            if (p.sparsity) reg += log1pf(2.0f * s * s);
-                            if (alpha >= a.alpha_skip) {
-                                wgt = T * alpha;
-                                push = true;
-                                S.add(sigma);
-                                T = __expf(-S.s);
-                                if (T < a.t_stop) ray.alive = false;
-                            }
-                        }
-                    }
-                }
-            }
-            uint64_t pm = __builtin_amdgcn_ballot_w64(push);
-            if (push) {
-                int slot = qn + __builtin_popcountll(pm & ((1ull << lane) - 1ull));
-                float* rw = queue + slot * kRowFloats;
-                *reinterpret_cast<float4*>(rw) = make_float4(px, py, pz, qx);
-                *reinterpret_cast<float4*>(rw + 4) =
-                    make_float4(qy, qz, wgt, __builtin_bit_cast(float, lane));
-            }
-            qn += __builtin_popcountll(pm);
-        }
-        if (qn == 0) break;  // nothing queued and every lane done
-
-        // ---------------- phase B: shade up to 64 queued rows -----------------------------------
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        const int nbatch = qn < 64 ? qn : 64;
-        if constexpr (PREC == kPrecFP32) {
-            const bool mine = lane < nbatch;
-            float w[8];
-            uint32_t off[8];
-            if (mine) {
-                float4 r0 = *reinterpret_cast<const float4*>(queue + lane * kRowFloats);
-                float4 r1 = *reinterpret_cast<const float4*>(queue + lane * kRowFloats + 4);
-                segment_corners<C, uint32_t>(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, n, w, off);
-            }
-            float z[Z];
-            if (mine) {
-                gather_integral<C>(a.voxels, w, off, z);
-            } else {
-#pragma unroll
-                for (int i = 0; i < Z; ++i) z[i] = 0.0f;
-            }
-            float b0[Z / 2], b1[Z / 2];
-            rows_to_b_f32<Z>(z, b0, b1);
-            shade_tile_f32<D>(a, queue, hvs, acc, 0, nbatch, b0, lane);
-            if (nbatch > 32) shade_tile_f32<D>(a, queue, hvs, acc, 1, nbatch, b1, lane);
-        } else {
-            shade_batch16<D, PREC>(a, queue, hvs, acc, nbatch, lane, n);
-        }
-        // drop the shaded rows: move rows [64, qn) down to [0, qn - 64)
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        const int rest = qn - nbatch;
-        const bool move = lane < rest;
-        float4 m0 = make_float4(0.f, 0.f, 0.f, 0.f), m1 = m0;
-        if (move) {
-            m0 = *reinterpret_cast<const float4*>(queue + (nbatch + lane) * kRowFloats);
-            m1 = *reinterpret_cast<const float4*>(queue + (nbatch + lane) * kRowFloats + 4);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        if (move) {
-            *reinterpret_cast<float4*>(queue + lane * kRowFloats) = m0;
-            *reinterpret_cast<float4*>(queue + lane * kRowFloats + 4) = m1;
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        qn = rest;
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    if (valid) {
-        const ReqDesc& rq = p.req[req];
-        reinterpret_cast<float4*>(rq.rgba)[out_idx] =
-            make_float4(acc[lane * 3], acc[lane * 3 + 1], acc[lane * 3 + 2], T);
-        if (rq.count) rq.count[out_idx] = used;
-        if (rq.sparsity) rq.sparsity[out_idx] = reg;
-    }
-}
 
 // FP32 matrix-core A operands (shade_tile_f32's order) from the canonical parameters of a decoder
 // without pre-layers: one float per (k step, lane).
@@ -584,7 +46,7 @@ hipError_t launch_pack_f32(int C, int H, int npost, const float* W, float* out, 
     return hipGetLastError();
 }
 
-static size_t render_tile_lds_bytes(int n, int H, int prec) {
+size_t render_tile_lds_bytes(int n, int H, int prec) {
     int nb = n >> 3;
     int64_t nwords = kBlocksInLds ? ((int64_t)nb * nb * nb + 31) >> 5 : 0;
     return (size_t)((nwords + 3) & ~3) * 4 + kWavesPerGroup * (size_t)wave_lds_floats(H, prec) * sizeof(float);
@@ -596,44 +58,16 @@ bool tile_fits(int n) {
     return n % 8 == 0 && (!kBlocksInLds || nwords <= kMaxLdsBlockWords);
 }
 
-template <class D, int PREC, bool RT>
-static hipError_t launch_t3(const BatchArgs& p, bool dch0, hipStream_t s) {
-    constexpr int threads = 64 * kWavesPerGroup;
-    const int64_t per_group = (int64_t)p.rays_per_wave * kWavesPerGroup;
-    dim3 grid((unsigned)((p.total + per_group - 1) / per_group)), block(threads);
-    size_t lds = render_tile_lds_bytes(p.g.n, D::H, PREC);
-    if (dch0)
-        hipLaunchKernelGGL((render_tile_kernel<D, PREC, RT, true>), grid, block, lds, s, p);
-    else
-        hipLaunchKernelGGL((render_tile_kernel<D, PREC, RT, false>), grid, block, lds, s, p);
-    return hipGetLastError();
-}
-
-template <class D, bool RT>
-static hipError_t launch_t2(const BatchArgs& p, int prec, bool dch0, hipStream_t s) {
-    if (prec == kPrecBF16) return launch_t3<D, kPrecBF16, RT>(p, dch0, s);
-    if (prec == kPrecFP16) return launch_t3<D, kPrecFP16, RT>(p, dch0, s);
-    return launch_t3<D, kPrecFP32, RT>(p, dch0, s);
-}
-
-template <class D>
-static hipError_t launch_t1(const BatchArgs& p, int prec, bool dch0, hipStream_t s) {
-    return p.g.traversal == 1 ? launch_t2<D, true>(p, prec, dch0, s)
-                              : launch_t2<D, false>(p, prec, dch0, s);
-}
-
 hipError_t launch_render_tiles(int variant, const BatchArgs& args, int prec, bool dch0,
                                hipStream_t s) {
     if (args.total == 0) return hipSuccess;
     if (prec != kPrecFP32 && !args.g.pack16) return hipErrorInvalidValue;  // no 16x16x32 operands
     BatchArgs p = args;
     p.rays_per_wave = rays_per_wave(p.total);
-    switch (variant) {
-        case 0: return launch_t1<DecB32>(p, prec, dch0, s);
-        case 1: return launch_t1<DecB64>(p, prec, dch0, s);
-        case 2: return launch_t1<DecB64C64>(p, prec, dch0, s);
-        default: return hipErrorInvalidValue;
-    }
+    const int64_t n1 = (int64_t)p.g.n + 1;
+    const bool b32 = n1 * n1 * n1 * p.g.channels * (int64_t)sizeof(float) <= (int64_t)UINT32_MAX;
+    return b32 ? launch_tiles<true>(variant, p, prec, dch0, s)
+               : launch_tiles_wide(variant, p, prec, dch0, s);
 }
 
 }  // namespace diver

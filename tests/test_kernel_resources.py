@@ -24,7 +24,7 @@ def report():
 
 
 def _tile_kernels(report):
-    ks = report['render_tile.hip']
+    ks = {**report['render_tile.hip'], **report['render_tile_wide.hip']}
     return {name: r for name, r in ks.items() if 'render_tile_kernel' in name}
 
 
@@ -34,8 +34,13 @@ def test_every_source_reported(report):
 
 
 def test_tile_kernels_instantiated(report):
-    # 3 decoder shapes x 3 precisions x 2 walks x 2 density forms
-    assert len(_tile_kernels(report)) == 36
+    # 3 decoder shapes x 3 precisions x 2 walks x 2 density forms x 2 feature addressings
+    # (32-bit byte offsets in render_tile.hip, 64-bit in render_tile_wide.hip)
+    assert len(_tile_kernels(report)) == 72
+    assert all(name.endswith('ELb1EEEvNS_9BatchArgsE') for name in report['render_tile.hip']
+               if 'render_tile_kernel' in name)
+    assert all(name.endswith('ELb0EEEvNS_9BatchArgsE') for name in report['render_tile_wide.hip']
+               if 'render_tile_kernel' in name)
 
 
 def test_matrix_core_tile_kernels_do_not_spill(report):
@@ -48,13 +53,16 @@ def test_matrix_core_tile_kernels_do_not_spill(report):
         if prec in '12' and dch0 == '1':
             assert r['vgpr_spill'] == 0 and r['scratch_bytes'] == 0, (name, r)
             checked += 1
-    assert checked == 12
+    assert checked == 24
 
 
-def test_benchmark_kernel_fits_three_waves_per_simd(report):
-    name = 'render_tile_kernelINS_3DecILi32ELi32ELi0ELb0ELi1ELi1EEELi1ELb1ELb1EE'
+def test_benchmark_kernel_fits_four_waves_per_simd(report):
+    # BF16, real-time walk, density from channel 0, 32-bit feature offsets: 16 waves per CU need
+    # at most 128 registers per lane (and the LDS of 16 waves, 7.75 KB each)
+    name = 'render_tile_kernelINS_3DecILi32ELi32ELi0ELb0ELi1ELi1EEELi1ELb1ELb1ELb1EE'
     (r,) = [v for k, v in _tile_kernels(report).items() if name in k]
-    assert r['vgpr'] + r['agpr'] <= 168 and r['vgpr_spill'] == 0, r
+    assert r['vgpr'] + r['agpr'] <= 128 and r['vgpr_spill'] == 0, r
+    assert r['lds_bytes'] == 0  # dynamic LDS only (the wave's queue, accumulators, view terms)
 
 
 def test_scratch_bounded_everywhere(report):
