@@ -29,6 +29,18 @@ struct RayWalk {
         sz = (float)sgnf(dz);
     }
 
+    // The voxel of the segment from the current point to (nx, ny, nz), int(min(l + 1e-4,
+    // n + 1e-4)) per axis (segment_voxel1): positions only move the way the direction points
+    // along each axis (every step adds d * t with t > 0, or snaps to the next plane ahead), so the
+    // smaller coordinate is the entry's where the direction points up and the exit's otherwise.
+    __device__ __forceinline__ void voxel(float nx, float ny, float nz, int& vx, int& vy,
+                                          int& vz) const {
+#pragma clang fp contract(off)
+        vx = (int)((dx > 0.0f ? lx : nx) + 1e-4f);
+        vy = (int)((dy > 0.0f ? ly : ny) + 1e-4f);
+        vz = (int)((dz > 0.0f ? lz : nz) + 1e-4f);
+    }
+
     // one exact DDA step from (px,py,pz)
     __device__ __forceinline__ void step(float px, float py, float pz, float& qx, float& qy,
                                          float& qz) const {

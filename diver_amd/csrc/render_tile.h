@@ -326,7 +326,7 @@ __device__ __forceinline__ void skip_empty(RayWalk& ray, const OccCache& occ, in
             dda_step(ray.lx, ray.ly, ray.lz, ray.dx, ray.dy, ray.dz, ray.sx, ray.sy, ray.sz, nx, ny,
                      nz);
         int vx, vy, vz;
-        segment_voxel1(ray.lx, ray.ly, ray.lz, nx, ny, nz, vx, vy, vz);
+        ray.voxel(nx, ny, nz, vx, vy, vz);
         const uint32_t ux = (uint32_t)vx, uy = (uint32_t)vy, uz = (uint32_t)vz;
         // (ux | uy | uz) < n implies each is below n (and is exact when n is a power of two)
         const bool inr = (ux | uy | uz) < (uint32_t)n;
@@ -414,7 +414,7 @@ __device__ __forceinline__ void phase_a(TileLane& L, const BatchArgs& p, const u
                     ray.alive = false;
                 } else {
                     int vx, vy, vz;
-                    segment_voxel1(ray.lx, ray.ly, ray.lz, nx, ny, nz, vx, vy, vz);
+                    ray.voxel(nx, ny, nz, vx, vy, vz);
                     px = ray.lx; py = ray.ly; pz = ray.lz;
                     ray.lx = nx; ray.ly = ny; ray.lz = nz;
                     if (L.occ.lookup(a.mask, blocks, a.bricks, n, vx, vy, vz)) {
