@@ -89,6 +89,7 @@ struct diver_decoder {
     uint32_t* pack16_bf16 = nullptr;  // matrix-core A operands (16x16x32 layout, tile kernel)
     uint32_t* pack16_f16 = nullptr;
     float* pack_f32 = nullptr;     // FP32 matrix-core A operands (repacked on every update)
+    float* pack_pre = nullptr;     // FP32 matrix-core A operands of the pre-layers (repacked on every update)
 };
 
 extern "C" {
@@ -313,6 +314,7 @@ int diver_decoder_create(const diver_decoder_desc* desc, const float* params_hos
         if (d->pack16_bf16) (void)hipFree(d->pack16_bf16);
         if (d->pack16_f16) (void)hipFree(d->pack16_f16);
         if (d->pack_f32) (void)hipFree(d->pack_f32);
+        if (d->pack_pre) (void)hipFree(d->pack_pre);
         delete d;
         return fail(code, "%s", what);
     };
@@ -361,6 +363,14 @@ int diver_decoder_create(const diver_decoder_desc* desc, const float* params_hos
         if (diver::launch_pack_f32(C, H, desc->n_post, d->W, d->pack_f32, nullptr) != hipSuccess ||
             hipStreamSynchronize(nullptr) != hipSuccess)
             return cleanup(DIVER_EHIP, "decoder FP32 pack failed");
+    } else if (!desc->pre_identity && desc->channels % 16 == 0 && desc->hidden % 32 == 0) {
+        const int64_t nf = diver::pack_pre_f32_size(desc->channels, desc->hidden, desc->n_pre);
+        if (hipMalloc(&d->pack_pre, sizeof(float) * nf) != hipSuccess)
+            return cleanup(DIVER_ENOMEM, "decoder pack allocation failed");
+        if (diver::launch_pack_pre_f32(desc->channels, desc->hidden, desc->n_pre, d->W, d->pack_pre,
+                                       nullptr) != hipSuccess ||
+            hipStreamSynchronize(nullptr) != hipSuccess)
+            return cleanup(DIVER_EHIP, "decoder FP32 pre-layer pack failed");
     }
     *out = d;
     return DIVER_OK;
@@ -388,8 +398,13 @@ int diver_decoder_update_range(diver_decoder* d, const float* params, int64_t fi
     int st = hip_status(hipMemcpyAsync(d->W + first, params, sizeof(float) * count,
                                        hipMemcpyDeviceToDevice, S(stream)),
                         "decoder_update");
-    if (st != DIVER_OK || !d->pack_f32) return st;
+    if (st != DIVER_OK) return st;
     // the FP32 operands follow W on the same stream
+    if (d->pack_pre)
+        return hip_status(diver::launch_pack_pre_f32(d->desc.channels, d->desc.hidden, d->desc.n_pre,
+                                                     d->W, d->pack_pre, S(stream)),
+                          "decoder_update (FP32 pre-layer pack)");
+    if (!d->pack_f32) return st;
     return hip_status(diver::launch_pack_f32(d->desc.channels, d->desc.hidden, d->desc.n_post, d->W,
                                              d->pack_f32, S(stream)),
                       "decoder_update (FP32 pack)");
@@ -406,6 +421,7 @@ int diver_decoder_destroy(diver_decoder* d) {
     if (d->pack16_bf16) (void)hipFree(d->pack16_bf16);
     if (d->pack16_f16) (void)hipFree(d->pack16_f16);
     if (d->pack_f32) (void)hipFree(d->pack_f32);
+    if (d->pack_pre) (void)hipFree(d->pack_pre);
     hipSetDevice(dev);
     delete d;
     return DIVER_OK;
@@ -652,7 +668,8 @@ int diver_alpha_map(const diver_grid* grid, const diver_decoder* dec, int64_t n_
     a.n_rays = n_rays;
     a.o = o;
     a.d = d;
-    return hip_status(diver::launch_alpha_map(dec->variant, a, pad, alpha_map, S(stream)),
+    return hip_status(diver::launch_alpha_map(dec->variant, a, pad, alpha_map, dec->pack_pre,
+                                              S(stream)),
                       "alpha_map");
 }
 

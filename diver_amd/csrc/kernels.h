@@ -102,7 +102,12 @@ hipError_t launch_mlp_eval(int variant, int64_t batch, const float* coord, const
                            uint8_t* finish, const float* W, float alpha_skip, float t_stop,
                            hipStream_t s);
 hipError_t launch_render(int variant, const RenderArgs& a, int prec, hipStream_t s);
-hipError_t launch_alpha_map(int variant, const RenderArgs& a, int pad, float* map, hipStream_t s);
+hipError_t launch_alpha_map(int variant, const RenderArgs& a, int pad, float* map,
+                            const float* pack_pre, hipStream_t s);
+// FP32 matrix-core A operands of a decoder's pre-layers (the occupancy kernel's density MLP),
+// packed on the device from its canonical parameters W: pack_pre_f32_size floats.
+int64_t pack_pre_f32_size(int C, int H, int npre);
+hipError_t launch_pack_pre_f32(int C, int H, int npre, const float* W, float* out, hipStream_t s);
 hipError_t launch_corner_keys(int n1, int64_t K, const float* ci, const float* co, int64_t* keys,
                               hipStream_t s);
 hipError_t launch_integrate_table(const float* table, int C, int64_t K, const int64_t* crow,

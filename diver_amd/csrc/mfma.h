@@ -88,6 +88,34 @@ __device__ __forceinline__ f32x16_t mfma32_f32(float a, float b, f32x16_t c) {
     return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
 }
 
+// acc += sum over k steps q < K of A(step q) B(bv[q]) with FP32 operands; the A operands (64
+// floats per step at pk) are loaded 8 steps at a time, the next 8 in flight during the current
+// products, which bounds the registers they hold.
+template <int K>
+__device__ __forceinline__ f32x16_t mfma_chain_f32(const float* __restrict__ pk, int lane,
+                                                   const float (&bv)[K], f32x16_t acc) {
+    static_assert(K % 8 == 0, "k steps in chunks of 8");
+    float w[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) w[i] = pk[i * 64 + lane];
+#pragma unroll
+    for (int k0 = 0; k0 < K; k0 += 8) {
+        float wn[8];
+        if (k0 + 8 < K) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) wn[i] = pk[(k0 + 8 + i) * 64 + lane];
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) acc = mfma32_f32(w[i], bv[k0 + i], acc);
+        __builtin_amdgcn_sched_barrier(0);
+        if (k0 + 8 < K) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) w[i] = wn[i];
+        }
+    }
+    return acc;
+}
+
 // Feature index of k step q, half h (0: lanes 0-31, 1: lanes 32-63) in the D-ordered k steps.
 __host__ __device__ constexpr int d_order_feature(int q, int h) {
     return 32 * (q / 16) + 8 * ((q % 16) / 4) + 4 * h + (q % 4);

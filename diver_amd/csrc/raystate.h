@@ -385,6 +385,89 @@ __device__ __forceinline__ bool camera_pixel(const CameraDesc& cam, int64_t r, i
     return k < tiles_y && x < cam.width && y < cam.height;
 }
 
+__device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0; }
+
+// Address of feature element `elem` of the grid.  B32: the grid's bytes fit 32 bits, so the
+// offset is formed in 32 bits and the load takes the grid pointer as its scalar base.
+template <bool B32>
+__device__ __forceinline__ const float* feature_at(const float* __restrict__ voxels, uint32_t elem) {
+    if constexpr (B32)
+        return reinterpret_cast<const float*>(reinterpret_cast<const char*>(voxels) + elem * 4u);
+    else
+        return voxels + elem;
+}
+
+// FP32 feature integral of gather_integral with feature_at addressing.
+template <int C, bool B32>
+__device__ __forceinline__ void gather_integral_at(const float* __restrict__ voxels, const float (&w)[8],
+                                                   const uint32_t (&off)[8], float (&f)[C]) {
+    static_assert(C % 4 == 0, "channel count must be a multiple of 4");
+#pragma unroll
+    for (int c = 0; c < C / 4; ++c) {
+        float4 q[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) q[j] = reinterpret_cast<const float4*>(feature_at<B32>(voxels, off[j]))[c];
+        float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            s0 += w[j] * q[j].x;
+            s1 += w[j] * q[j].y;
+            s2 += w[j] * q[j].z;
+            s3 += w[j] * q[j].w;
+        }
+        f[4 * c + 0] = s0;
+        f[4 * c + 1] = s1;
+        f[4 * c + 2] = s2;
+        f[4 * c + 3] = s3;
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+constexpr int kSkip = 2;           // cheap steps per walk iteration over voxels known to be empty (0: off)
+
+// Cheap steps over voxels already known to be empty.  Before each full walk iteration a lane
+// takes exact DDA steps for as long as the new voxel lies inside the grid, in its cached 4^3
+// brick (or in the cached brick's 8^3 block when that block is empty) and has its occupancy bit
+// clear: the full iteration would only have stepped over such a voxel.  The region test compares
+// the voxel's region coordinates, packed 10 bits an axis, with the cached region's (n <= 4096).
+// A step that fails the test is not taken here; the full iteration takes it again (same operands,
+// same result).  Up to kSkip steps per call (more lets lanes that cannot skip wait longer); lanes
+// stop one by one.
+template <bool RT, bool AF>
+__device__ __forceinline__ void skip_empty(RayWalk& ray, const OccCache& occ, int n) {
+    const float nf = (float)n;
+    bool sk = ray.alive && occ.brick >= 0;
+    const uint32_t sh = occ.shift;
+#pragma unroll 1
+    for (int k = 0; k < kSkip; ++k) {
+        // a wave whose every ray takes the reciprocal step needs no step bound: each step moves
+        // one axis to its next integer plane ahead, so a walk leaves the grid after <= 3(N+1)
+        if constexpr (!AF) sk = sk && ray.steps < ray.max_steps;
+        if (!wave_any(sk)) break;
+        float nx, ny, nz;
+        dda_step_r(ray.lx, ray.ly, ray.lz, ray.dx, ray.dy, ray.dz, ray.rx, ray.ry, ray.rz, ray.sx,
+                   ray.sy, ray.sz, nx, ny, nz);
+        if (!AF && !ray.fast)
+            dda_step(ray.lx, ray.ly, ray.lz, ray.dx, ray.dy, ray.dz, ray.sx, ray.sy, ray.sz, nx, ny,
+                     nz);
+        int vx, vy, vz;
+        ray.voxel(nx, ny, nz, vx, vy, vz);
+        const uint32_t ux = (uint32_t)vx, uy = (uint32_t)vy, uz = (uint32_t)vz;
+        // (ux | uy | uz) < n implies each is below n (and is exact when n is a power of two)
+        const bool inr = (ux | uy | uz) < (uint32_t)n;
+        const uint32_t key = ((uz >> sh) << 20) | ((uy >> sh) << 10) | (ux >> sh);
+        const uint32_t bit = ((uz & 3u) << 4) | ((uy & 3u) << 2) | (ux & 3u);
+        // an empty block's brick word is zero, so the bit test covers both regions
+        const bool set = ((bit < 32u ? occ.lo : occ.hi) >> (bit & 31u)) & 1u;
+        sk = sk && inr && key == occ.key && !set &&
+             !(RT ? outside_open(nx, ny, nz, nf) : outside(nx, ny, nz, nf));
+        ray.lx = sk ? nx : ray.lx;
+        ray.ly = sk ? ny : ray.ly;
+        ray.lz = sk ? nz : ray.lz;
+        if constexpr (!AF) ray.steps += sk ? 1 : 0;
+    }
+}
+
 struct Kahan {
     float s = 0.0f, c = 0.0f;
     __device__ __forceinline__ void add(float x) {
@@ -396,6 +479,5 @@ struct Kahan {
     }
 };
 
-__device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0; }
 
 }  // namespace diver

@@ -33,9 +33,11 @@ class _WalkGrid:
 
 
 def alpha_map(grid: SceneGrid, decoder, o: torch.Tensor, d: torch.Tensor, pad: int = 0,
-              batch: int = 1 << 20, masked: bool = True) -> torch.Tensor:
+              batch: int = 1 << 24, masked: bool = True) -> torch.Tensor:
     """[N, N, N] float32 map (indexed [z][y][x]) of the largest rendering weight any ray gives
-    each voxel (prune.py:83-137).  masked=False walks every voxel (a model without a mask)."""
+    each voxel (prune.py:83-137).  masked=False walks every voxel (a model without a mask).
+    The kernel keeps no per-ray state in memory, so a batch only bounds one launch: large batches
+    amortise the launch's tail of long rays (the map is the same for any batching)."""
     n = grid.n
     walk = grid if masked else _WalkGrid(grid, torch.ones_like(grid.mask))
     out = torch.zeros(n ** 3, dtype=torch.float32, device=grid.voxels.device)

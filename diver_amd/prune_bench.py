@@ -62,6 +62,7 @@ def main(argv=None):
     ap.add_argument('--views', type=int, default=4)
     ap.add_argument('--reps', type=int, default=3)
     ap.add_argument('--no-plain', action='store_true')
+    ap.add_argument('--batch', type=int, default=1 << 24, help='rays per alpha-map launch')
     args = ap.parse_args(argv)
     dev = torch.device('cuda', 0)
     sc = S.config2_scene()
@@ -74,12 +75,12 @@ def main(argv=None):
     d = torch.cat([x[1] for x in od]).contiguous()
     R = o.shape[0]
 
-    prune.alpha_map(grid, dec, o[:65536], d[:65536])
+    prune.alpha_map(grid, dec, o[:65536], d[:65536], batch=args.batch)
     torch.cuda.synchronize()
     best = float('inf')
     for _ in range(args.reps):
         t0 = time.perf_counter()
-        fused = prune.alpha_map(grid, dec, o, d)
+        fused = prune.alpha_map(grid, dec, o, d, batch=args.batch)
         torch.cuda.synchronize()
         best = min(best, time.perf_counter() - t0)
     _, num, _ = calls.masked_intersect_raw(o, d, grid.mask, grid.xyzmin, grid.n, grid.voxel_size,
@@ -88,7 +89,7 @@ def main(argv=None):
     alg_bytes = R * 24 + intervals * (8 * grid.channels * 4 + 4)
     res = {'workload': f'alpha map (prune.py:83-137), {args.views} views of 800x800 rays, 256^3 x 32 '
                        'grid, DIVeR32 decoder FP32, offline walk over the mask',
-           'rays': R, 'intervals': intervals,
+           'rays': R, 'intervals': intervals, 'rays_per_launch': min(R, args.batch),
            'fused': {'ms': round(best * 1e3, 3), 'rays_per_s': round(R / best, 1),
                      'achieved_GBps': round(alg_bytes / best / 1e9, 1)},
            'voxels_over_1e-2': int((fused > 1e-2).sum().item())}
