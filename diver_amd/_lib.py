@@ -25,6 +25,7 @@ EXPORTS = [
     'diver_render_rays_sparsity', 'diver_server_clock', 'diver_server_create',
     'diver_server_destroy', 'diver_server_add_scene', 'diver_server_submit', 'diver_server_wait',
     'diver_server_take_completed', 'diver_server_history', 'diver_adam_step_rows',
+    'diver_adjoint_decoder_forward', 'diver_adjoint_decoder_backward', 'diver_positional_encoding',
 ]
 
 # status codes of include/diver.h
@@ -127,6 +128,10 @@ def _declare(L):
                             i32, vp],
         'diver_adam_step_rows': [vp, vp, vp, vp, i64, vp, i64, i32, ctypes.c_double,
                                  ctypes.c_double, fl, fl, fl, fl, i32, vp],
+        'diver_adjoint_decoder_forward': [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp],
+        'diver_adjoint_decoder_backward': [vp, i64, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                                           vp, vp, vp, vp],
+        'diver_positional_encoding': [i64, vp, vp, vp],
     }
     sig.update({
         'diver_server_create': [ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)],

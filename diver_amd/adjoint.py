@@ -11,13 +11,17 @@ log(1 + 2 s^2) per ray, and its adjoint adds g / (B M) * 4 s / (1 + 2 s^2) to ea
 Backward (for decoders without pre-layers, e.g. the benchmark decoder):
   1. re-walk every ray and recompute each composited interval's integral, density and
      transmittance (``diver_render_rows``);
-  2. compositing adjoint, per interval k of a ray with C = sum_i T_i a_i c_i + bg T_n:
+  2. the decoder over the rows on the matrix cores (``diver_adjoint_decoder_forward``: relu(y1),
+     relu(y2), colour c and weight w = T a per row), then the compositing adjoint, per interval k
+     of a ray with C = sum_i T_i a_i c_i + bg T_n:
         dC/dc_k     = T_k a_k
         dC/dsigma_k = T_{k+1} c_k - (sum_{i>k} T_i a_i c_i + bg T_n)
-     (the suffix sums come from a float64 segmented cumsum over the rows);
-  3. decoder adjoint as batched FP32 GEMMs over the rows (canonical form, include/diver.h):
-        do = dc * c (1 - c);  dW_o = do^T r2;  dr2 = (do W_o) * [y2 > 0];  dW_2 = dr2^T r1
-        dr1 = (dr2 W_2) * [y1 > 0];  dW_c = dr1^T F;  dW_v = dr1^T PE(d);  dF = dr1 W_c + ds w_s
+     (the per-ray suffix sums in FP64, ``diver_ray_suffix``);
+  3. the decoder's backward pass on the matrix cores (``diver_adjoint_decoder_backward``):
+        do = dc * c (1 - c);  dr2 = (do W_o) * [y2 > 0];  dr1 = (dr2 W_2) * [y1 > 0];
+        dF = dr1 W_c + ds w_s
+     and the weight gradients as batched GEMM reductions over the rows:
+        dW_o = do^T r2;  dW_2 = dr2^T r1;  dW_c = dr1^T F;  dW_v = (per-ray sums of dr1)^T PE(d)
   4. grid adjoint: ``diver_integrate_backward`` scatters dF through the 8 corner weights.
 The reference obtains the same gradient from torch autograd over stored K x C / B x M
 activations (model/diver.py:141-207, utils/integrator.py:62-113, train.py:102-113).
@@ -129,42 +133,27 @@ class _FusedRender(Function):
         if Kp > K:
             ray = torch.cat([ray, torch.zeros(Kp - K, dtype=ray.dtype, device=dev)])
         pin, pout = rows[:K, 0:3], rows[:K, 3:6]
-        s, Tb, a, F = rows[:, 6], rows[:, 7], rows[:, 8], rows[:, 9:].contiguous()
-        ws, bs, Wc, Wv, bc, W2, b2, Wo, bo = split_canonical(params, C, H)
-        pe = positional_encoding(d)[ray]
-        # decoder forward on the rows
-        y1 = F @ Wc.T + pe @ Wv.T + bc
-        r1 = torch.relu(y1)
-        y2 = r1 @ W2.T + b2
-        r2 = torch.relu(y2)
-        c = torch.sigmoid(r2 @ Wo.T + bo)
-        w = Tb * a
+        F = rows[:, 9:]
+        # the decoder over the rows on the matrix cores: relu(y1), relu(y2), colour, weight
+        r1, r2, c, w = calls.adjoint_decoder_forward(decoder, rows, ray, d)
         # suffix sums within each ray: sum over the later rows of w c, plus the background
         # (one lane per ray, FP64 sums)
         suffix = calls.ray_suffix(row_off, count, w[:K], c[:K],
                                   rgba[:, 3].contiguous() if ctx.white_back else None)
-        if Kp > K:
-            suffix = torch.nn.functional.pad(suffix, (0, 0, 0, Kp - K))
-        gk = g.contiguous()[ray]
-        t_after = Tb * (1.0 - a)
-        dsigma = (gk * (t_after[:, None] * c - suffix)).sum(1)
-        act = decoder.canon.density_act
-        ds = dsigma * (torch.sigmoid(s) if act == 1 else (s > 0).to(s.dtype))
-        if ctx.sparsity and g_reg is not None:
-            # d/ds log(1 + s^2 / 0.5) = 4 s / (1 + 2 s^2); padding rows have s = 0
-            ds = ds + (g_reg / bm) * (4.0 * s / (1.0 + 2.0 * s * s))
-        dc = gk * w[:, None]
-        do = dc * c * (1.0 - c)
+        # sparsity term: d/ds of g_reg * sum log(1 + s^2 / 0.5) / (B M), applied in the kernel
+        sscale = (g_reg / bm) if (ctx.sparsity and g_reg is not None) else None
+        dF, do, ds, dr1, dr2 = calls.adjoint_decoder_backward(decoder, rows, ray, K, g.contiguous(),
+                                                              suffix, r1, r2, c, w, sscale)
+        # weight gradients: reductions over every row
         dWo = _wgrad(do, r2)
         dbo = do.sum(0)
-        dr2 = (do @ Wo) * (y2 > 0)
         dW2 = _wgrad(dr2, r1)
         db2 = dr2.sum(0)
-        dr1 = (dr2 @ W2) * (y1 > 0)
         dWc = _wgrad(dr1, F)
-        dWv = _wgrad(dr1, pe)
         dbc = dr1.sum(0)
-        dF = dr1 @ Wc + ds[:, None] * ws[None, :]
+        # dW_v = sum over rows of dr1 (x) PE(d of the row's ray) = (per-ray sums of dr1)^T PE(d)
+        seg = torch.zeros((R, H), dtype=torch.float32, device=dev).index_add_(0, ray[:K], dr1[:K])
+        dWv = seg.T @ calls.positional_encoding(d)
         dws = _wgrad(ds[:, None], F)[0]
         dbs = ds.sum().reshape(1)
         gp = torch.cat([dws, dbs, dWc.reshape(-1), dWv.reshape(-1), dbc, dW2.reshape(-1), db2,
@@ -174,10 +163,10 @@ class _FusedRender(Function):
             leaf = ctx.accumulate
             if leaf.grad is None:
                 leaf.grad = torch.zeros_like(leaf)
-            calls.integrate_backward(ctx.vshape, pin, pout, dF[:K].contiguous(), leaf.grad)
+            calls.integrate_backward(ctx.vshape, pin, pout, dF, leaf.grad)
             gv = None
         else:
-            gv = calls.integrate_backward(ctx.vshape, pin, pout, dF[:K].contiguous())
+            gv = calls.integrate_backward(ctx.vshape, pin, pout, dF)
         return gv, gp, None, None, None, None, None, None, None, None, None
 
 

@@ -21,7 +21,7 @@ BUILD = os.path.join(HERE, '_build')
 ARCH = os.environ.get('DIVER_OFFLOAD_ARCH', 'gfx950')
 
 SOURCES = ['traverse.hip', 'integrate.hip', 'composite.hip', 'rt_decoder.hip', 'render.hip',
-           'occupancy.hip', 'render_tile.hip', 'render_tile_wide.hip', 'adjoint.hip', 'prune.hip', 'optim.hip', 'capi.cpp', 'server.cpp']
+           'occupancy.hip', 'render_tile.hip', 'render_tile_wide.hip', 'adjoint.hip', 'adjoint_mlp.hip', 'prune.hip', 'optim.hip', 'capi.cpp', 'server.cpp']
 
 COMMON = ['-O3', '-fPIC', '-std=c++17', f'--offload-arch={ARCH}',
           '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',

@@ -252,6 +252,64 @@ def ray_suffix(row_offset, count, weight, color, background=None):
     return out
 
 
+def adjoint_decoder_forward(decoder, rows, ray, d):
+    """Forward half of the decoder adjoint over the rows of render_rows (diver_adjoint_decoder_
+    forward): (r1 [K, H], r2 [K, H], color [K, 3], weight [K]) for K = rows.shape[0]."""
+    dev = _dev(rows, ray, d)
+    rows, d = _f32(rows, 'rows'), _f32(d, 'd')
+    if ray.dtype != torch.int64 or ray.numel() != rows.shape[0]:
+        raise TypeError('ray must be int64 with one entry per row')
+    ray = ray.contiguous()
+    K, H = rows.shape[0], decoder.hidden
+    r1 = torch.empty((K, H), dtype=torch.float32, device=dev)
+    r2 = torch.empty((K, H), dtype=torch.float32, device=dev)
+    color = torch.empty((K, 3), dtype=torch.float32, device=dev)
+    weight = torch.empty((K,), dtype=torch.float32, device=dev)
+    _lib.call('diver_adjoint_decoder_forward', decoder.handle, K, rows.data_ptr(), ray.data_ptr(),
+              d.data_ptr(), r1.data_ptr(), r2.data_ptr(), color.data_ptr(), weight.data_ptr(),
+              _s(rows))
+    return r1, r2, color, weight
+
+
+def adjoint_decoder_backward(decoder, rows, ray, n_valid, grad_rgb, suffix, r1, r2, color, weight,
+                             sparsity_scale=None):
+    """Backward half of the decoder adjoint (diver_adjoint_decoder_backward): (d_feature
+    [n_valid, C], d_out [K, 3], d_s [K], d_r1 [K, H], d_r2 [K, H]).  sparsity_scale: a 0-dim
+    device tensor (dL/d(sigma_loss) / (B M)) or None."""
+    tensors = [rows, ray, grad_rgb, suffix, r1, r2, color, weight]
+    dev = _dev(*tensors)
+    rows, g, suffix = _f32(rows, 'rows'), _f32(grad_rgb, 'grad_rgb'), _f32(suffix, 'suffix')
+    r1, r2, color, weight = (_f32(x, n) for x, n in ((r1, 'r1'), (r2, 'r2'), (color, 'color'),
+                                                       (weight, 'weight')))
+    ray = ray.contiguous()
+    K, H, C = rows.shape[0], decoder.hidden, decoder.channels
+    n_valid = int(n_valid)
+    if not 0 <= n_valid <= K or suffix.shape[0] < n_valid:
+        raise ValueError('need 0 <= n_valid <= rows and suffix rows >= n_valid')
+    sc = None
+    if sparsity_scale is not None:
+        sc = _f32(sparsity_scale.reshape(1), 'sparsity_scale')
+    d_feature = torch.empty((n_valid, C), dtype=torch.float32, device=dev)
+    d_out = torch.empty((K, 3), dtype=torch.float32, device=dev)
+    d_s = torch.empty((K,), dtype=torch.float32, device=dev)
+    d_r1 = torch.empty((K, H), dtype=torch.float32, device=dev)
+    d_r2 = torch.empty((K, H), dtype=torch.float32, device=dev)
+    _lib.call('diver_adjoint_decoder_backward', decoder.handle, K, n_valid, rows.data_ptr(),
+              ray.data_ptr(), g.data_ptr(), suffix.data_ptr(), _lib.ptr(sc), r1.data_ptr(),
+              r2.data_ptr(), color.data_ptr(), weight.data_ptr(), d_feature.data_ptr(),
+              d_out.data_ptr(), d_s.data_ptr(), d_r1.data_ptr(), d_r2.data_ptr(), _s(rows))
+    return d_feature, d_out, d_s, d_r1, d_r2
+
+
+def positional_encoding(d):
+    """PE(d) [R, 27] as the fused kernels evaluate it (diver_positional_encoding)."""
+    dev = _dev(d)
+    d = _f32(d, 'd')
+    pe = torch.empty((d.shape[0], 27), dtype=torch.float32, device=dev)
+    _lib.call('diver_positional_encoding', d.shape[0], d.data_ptr(), pe.data_ptr(), _s(d))
+    return pe
+
+
 def render_rows(grid, decoder, o, d, opts, count, row_offset, n_rows, capacity=None):
     """Recompute pass of the adjoint: rows [n_rows, 9 + C] (see include/diver.h).  With capacity
     > n_rows the tensor has capacity rows and rows n_rows.. are zero."""

@@ -1,0 +1,312 @@
+// Decoder and compositing adjoint over the recompute pass's rows, on the matrix cores.
+//
+// The rows of diver_render_rows ([p_in, p_out, s, T_before, alpha, F], one per composited interval,
+// grouped by ray in walk order) feed two kernels, 32 rows per tile and one wave per tile:
+//
+//   adj_forward_kernel: the colour decoder of each row, as the fused renderer evaluates it (FP32
+//     operands on v_mfma_f32_32x32x2_f32, weights as A, rows as B):
+//        y1 = W_c F + (W_v PE(d) + b_c);  r1 = relu(y1);  y2 = W_2 r1 + b_2;  r2 = relu(y2);
+//        c = sigmoid(W_o r2 + b_o);  w = T_before * alpha
+//     and writes r1, r2, c and w.  Per-ray suffix sums of w c (diver_ray_suffix) follow.
+//   adj_backward_kernel: the compositing adjoint of C = sum_k T_k a_k c_k + bg T_n and the
+//     decoder's backward pass,
+//        dsigma = g . (T_after c - suffix);  ds = dsigma act'(s) (+ the sparsity term);
+//        do = (g w) c (1 - c);  dr2 = (W_o^T do) [r2 > 0];  dr1 = (W_2^T dr2) [r1 > 0];
+//        dF = W_c^T dr1 + ds w_s
+//     with the transposed weights as A operands (packed by adj_pack_kernel, in the D order of the
+//     previous product, so each D tile is the next product's B operand in place), and writes dF
+//     (the grid adjoint's input), do, ds, dr1 and dr2 (the weight gradients' operands).
+// The weight gradients are reductions over all rows (dW_o = do^T r2, dW_2 = dr2^T r1, ...) and run
+// as batched GEMMs on the host side.
+#include "common.h"
+#include "decoder.h"
+#include "kernels.h"
+#include "mfma.h"
+
+namespace diver {
+
+// A-operand steps of the adjoint, 64 floats each: W_o^T (natural k pairs over the 3 outputs,
+// padded to 4), W_2^T and W_c^T (k steps in the D order of the product they follow).
+template <class D>
+struct AdjPack {
+    static constexpr int MT = D::H / 32, KH = D::H / 2;
+    static constexpr int wo = 0;                              // MT x 2 steps
+    static constexpr int w2 = wo + MT * 2;                     // MT x KH steps
+    static constexpr int wc = w2 + D::NPOST * MT * KH;         // (C / 32) x KH steps
+    static constexpr int steps = wc + (D::C / 32) * KH;
+};
+
+int64_t pack_adj_size(int C, int H, int npost) {
+    return 64 * ((int64_t)(H / 32) * 2 + (int64_t)npost * (H / 32) * (H / 2) + (int64_t)(C / 32) * (H / 2));
+}
+
+__global__ void __launch_bounds__(256) adj_pack_kernel(int C, int H, int npost,
+                                                       const float* __restrict__ W,
+                                                       float* __restrict__ out, int64_t total) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    const int r = (int)(idx & 31), h = (int)((idx >> 5) & 1);
+    int64_t step = idx >> 6;
+    const int MT = H / 32, KH = H / 2;
+    const int64_t wc_off = C + 1;                                   // w_s[C], b_s, then W_c[H][C]
+    const int64_t post0 = wc_off + (int64_t)H * C + (int64_t)H * kPE + H;
+    const int64_t out0 = post0 + (int64_t)npost * (H * H + H);
+    float v;
+    if (step < 2 * MT) {
+        const int mt = (int)(step / 2), kc = (int)(step % 2);
+        const int k = 2 * kc + h;
+        v = k < 3 ? W[out0 + (int64_t)k * H + 32 * mt + r] : 0.0f;
+    } else if ((step -= 2 * MT) < (int64_t)npost * MT * KH) {
+        const int mt = (int)(step / KH), q = (int)(step % KH);
+        v = W[post0 + (int64_t)d_order_feature(q, h) * H + 32 * mt + r];
+    } else {
+        step -= (int64_t)npost * MT * KH;
+        const int mt = (int)(step / KH), q = (int)(step % KH);
+        v = W[wc_off + (int64_t)d_order_feature(q, h) * C + 32 * mt + r];
+    }
+    out[idx] = v;
+}
+
+hipError_t launch_pack_adj(int C, int H, int npost, const float* W, float* out, hipStream_t s) {
+    if (C % 32 || H % 32 || npost != 1) return hipErrorInvalidValue;
+    const int64_t total = pack_adj_size(C, H, npost);
+    hipLaunchKernelGGL(adj_pack_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, C, H,
+                       npost, W, out, total);
+    return hipGetLastError();
+}
+
+// D-layout unit of register i of m tile mt on this lane half: 32 mt + 8 (i / 4) + 4 h + i % 4, so
+// registers 4 g .. 4 g + 3 are the units 32 mt + 8 g + 4 h + (0..3), one float4 of a row.
+__device__ __forceinline__ int d_unit(int mt, int i, int h) { return 32 * mt + 8 * (i >> 2) + 4 * h + (i & 3); }
+
+template <int MT>
+__device__ __forceinline__ void store_d_rows(float* __restrict__ dst, const f32x16_t (&v)[MT], int h,
+                                             bool relu) {
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            float4 x = make_float4(v[mt][4 * g], v[mt][4 * g + 1], v[mt][4 * g + 2], v[mt][4 * g + 3]);
+            if (relu) {
+                x.x = fmaxf(x.x, 0.0f);
+                x.y = fmaxf(x.y, 0.0f);
+                x.z = fmaxf(x.z, 0.0f);
+                x.w = fmaxf(x.w, 0.0f);
+            }
+            *reinterpret_cast<float4*>(dst + 32 * mt + 8 * g + 4 * h) = x;
+        }
+}
+
+template <class D>
+__global__ void __launch_bounds__(64) adj_forward_kernel(AdjArgs p) {
+    static_assert(D::NPRE == 0 && D::NPOST == 1 && D::C % 32 == 0 && D::H % 32 == 0, "decoder shape");
+    constexpr int C = D::C, H = D::H, MT = H / 32, KC = C / 2, KH = H / 2, HC = C / 2;
+    const int lane = threadIdx.x, c = lane & 31, h = lane >> 5;
+    const float* W = p.W;
+    const float* pk = p.pack;
+    for (int64_t tile = blockIdx.x; tile * 32 < p.n_rows; tile += gridDim.x) {
+        // per-tile opaque pointers keep the operand and weight loads inside the tile loop
+        asm volatile("" : "+s"(pk));
+        asm volatile("" : "+s"(W));
+        const int64_t row = tile * 32 + c;
+        const bool live = row < p.n_rows;
+        const int64_t rr = live ? row : 0;
+        const float* rw = p.rows + rr * p.row_stride;
+        const int64_t ray = p.ray[rr];
+        float pe[kPE];
+        positional_encoding(p.d[3 * ray], p.d[3 * ray + 1], p.d[3 * ray + 2], pe);
+        // y1 starts at this lane's units of W_v PE(d) + b_c
+        f32x16_t y[MT];
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const int u = d_unit(mt, i, h);
+                float a = W[D::bc_off + u];
+#pragma unroll
+                for (int j = 0; j < kPE; ++j) a += W[D::wv_off + u * kPE + j] * pe[j];
+                y[mt][i] = a;
+            }
+        // B operands of the head: lane c + 32 h holds channels [h C/2, (h + 1) C/2) of row c;
+        // one swap per pair of k steps puts (2 kc, 2 kc + 1) on the two lane halves
+        float g[HC];
+#pragma unroll
+        for (int i = 0; i < HC; ++i) g[i] = rw[9 + h * HC + i];
+        float b[KC];
+#pragma unroll
+        for (int kc = 0; kc < HC / 2; ++kc) {
+            float x = g[2 * kc], z = g[2 * kc + 1];
+            swap32f(x, z);
+            b[kc] = x;
+            b[kc + HC / 2] = z;
+        }
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) y[mt] = mfma_chain_f32<KC>(pk + mt * KC * 64, lane, b, y[mt]);
+        if (live) store_d_rows<MT>(p.r1 + rr * H, y, h, true);
+        float e[KH];
+#pragma unroll
+        for (int q = 0; q < KH; ++q) e[q] = fmaxf(y[q / 16][q % 16], 0.0f);
+        const int pofs = MT * KC * 64;
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+            f32x16_t s;
+#pragma unroll
+            for (int i = 0; i < 16; ++i) s[i] = W[D::post_off(0) + H * H + d_unit(mt, i, h)];
+            y[mt] = mfma_chain_f32<KH>(pk + pofs + mt * KH * 64, lane, e, s);
+        }
+        if (live) store_d_rows<MT>(p.r2 + rr * H, y, h, true);
+#pragma unroll
+        for (int q = 0; q < KH; ++q) e[q] = fmaxf(y[q / 16][q % 16], 0.0f);
+        f32x16_t o = {};
+        o = mfma_chain_f32<KH>(pk + pofs + MT * KH * 64, lane, e, o);
+        // outputs 0..2 of row c sit in registers 0..2 of lanes 0-31
+        if (h == 0 && live) {
+            const float* bo = W + D::out_off + 3 * H;
+            p.color[3 * rr + 0] = sigmoidf(o[0] + bo[0]);
+            p.color[3 * rr + 1] = sigmoidf(o[1] + bo[1]);
+            p.color[3 * rr + 2] = sigmoidf(o[2] + bo[2]);
+            p.weight[rr] = rw[7] * rw[8];
+        }
+    }
+}
+
+template <class D>
+__global__ void __launch_bounds__(64) adj_backward_kernel(AdjArgs p) {
+    static_assert(D::NPRE == 0 && D::NPOST == 1 && D::C % 32 == 0 && D::H % 32 == 0, "decoder shape");
+    constexpr int C = D::C, H = D::H, MT = H / 32, KH = H / 2, CT = C / 32;
+    using P = AdjPack<D>;
+    const int lane = threadIdx.x, c = lane & 31, h = lane >> 5;
+    const float* W = p.W;
+    const float* pa = p.pack_adj;
+    const float sscale = p.sparsity_scale ? *p.sparsity_scale : 0.0f;
+    for (int64_t tile = blockIdx.x; tile * 32 < p.n_rows; tile += gridDim.x) {
+        asm volatile("" : "+s"(pa));
+        asm volatile("" : "+s"(W));
+        const int64_t row = tile * 32 + c;
+        const bool live = row < p.n_rows;
+        const bool valid = row < p.n_valid;
+        const int64_t rr = live ? row : 0;
+        const float* rw = p.rows + rr * p.row_stride;
+        const float s = rw[6], T = rw[7], alpha = rw[8];
+        const float c0 = p.color[3 * rr], c1 = p.color[3 * rr + 1], c2 = p.color[3 * rr + 2];
+        const float wgt = p.weight[rr];
+        const int64_t ray = p.ray[rr];
+        const float g0 = p.g[3 * ray], g1 = p.g[3 * ray + 1], g2 = p.g[3 * ray + 2];
+        float u0 = 0.0f, u1 = 0.0f, u2 = 0.0f;
+        if (valid) {
+            u0 = p.suffix[3 * rr];
+            u1 = p.suffix[3 * rr + 1];
+            u2 = p.suffix[3 * rr + 2];
+        }
+        // compositing adjoint (every lane for its row c; both lane halves alike)
+        const float t_after = T * (1.0f - alpha);
+        const float dsigma = g0 * (t_after * c0 - u0) + g1 * (t_after * c1 - u1) + g2 * (t_after * c2 - u2);
+        float ds = dsigma * (D::DACT == 1 ? sigmoidf(s) : (s > 0.0f ? 1.0f : 0.0f));
+        // d/ds log(1 + s^2 / 0.5) = 4 s / (1 + 2 s^2); padding rows have s = 0
+        ds += sscale * (4.0f * s / (1.0f + 2.0f * s * s));
+        const float do0 = g0 * wgt * c0 * (1.0f - c0);
+        const float do1 = g1 * wgt * c1 * (1.0f - c1);
+        const float do2 = g2 * wgt * c2 * (1.0f - c2);
+        // dr2 = W_o^T do: k pairs (do0, do1), (do2, 0) over the lane halves
+        const float bA = h ? do1 : do0, bB = h ? 0.0f : do2;
+        f32x16_t t[MT];
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+            f32x16_t acc = {};
+            acc = mfma32_f32(pa[(P::wo + 2 * mt) * 64 + lane], bA, acc);
+            acc = mfma32_f32(pa[(P::wo + 2 * mt + 1) * 64 + lane], bB, acc);
+            // mask by r2 > 0 (r2 = relu(y2) written by the forward, in the same units)
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const float4 r = *reinterpret_cast<const float4*>(p.r2 + rr * H + 32 * mt + 8 * g + 4 * h);
+                acc[4 * g + 0] = r.x > 0.0f ? acc[4 * g + 0] : 0.0f;
+                acc[4 * g + 1] = r.y > 0.0f ? acc[4 * g + 1] : 0.0f;
+                acc[4 * g + 2] = r.z > 0.0f ? acc[4 * g + 2] : 0.0f;
+                acc[4 * g + 3] = r.w > 0.0f ? acc[4 * g + 3] : 0.0f;
+            }
+            t[mt] = acc;
+        }
+        if (live) store_d_rows<MT>(p.dr2 + rr * H, t, h, false);
+        float e[KH];
+#pragma unroll
+        for (int q = 0; q < KH; ++q) e[q] = t[q / 16][q % 16];
+        // dr1 = W_2^T dr2, masked by r1 > 0
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+            f32x16_t acc = {};
+            acc = mfma_chain_f32<KH>(pa + (P::w2 + mt * KH) * 64, lane, e, acc);
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const float4 r = *reinterpret_cast<const float4*>(p.r1 + rr * H + 32 * mt + 8 * g + 4 * h);
+                acc[4 * g + 0] = r.x > 0.0f ? acc[4 * g + 0] : 0.0f;
+                acc[4 * g + 1] = r.y > 0.0f ? acc[4 * g + 1] : 0.0f;
+                acc[4 * g + 2] = r.z > 0.0f ? acc[4 * g + 2] : 0.0f;
+                acc[4 * g + 3] = r.w > 0.0f ? acc[4 * g + 3] : 0.0f;
+            }
+            t[mt] = acc;
+        }
+        if (live) store_d_rows<MT>(p.dr1 + rr * H, t, h, false);
+#pragma unroll
+        for (int q = 0; q < KH; ++q) e[q] = t[q / 16][q % 16];
+        // dF = W_c^T dr1 + ds w_s
+        f32x16_t f[CT];
+#pragma unroll
+        for (int mt = 0; mt < CT; ++mt) {
+            f32x16_t acc;
+#pragma unroll
+            for (int i = 0; i < 16; ++i) acc[i] = ds * W[D::ws_off + d_unit(mt, i, h)];
+            f[mt] = mfma_chain_f32<KH>(pa + (P::wc + mt * KH) * 64, lane, e, acc);
+        }
+        if (valid) store_d_rows<CT>(p.dF + rr * C, f, h, false);
+        if (h == 0 && live) {
+            p.dout[3 * rr + 0] = do0;
+            p.dout[3 * rr + 1] = do1;
+            p.dout[3 * rr + 2] = do2;
+            p.ds[rr] = ds;
+        }
+    }
+}
+
+// PE(d) per ray (the fused kernels' convention), [R][27].
+__global__ void __launch_bounds__(256) pe_kernel(int64_t R, const float* __restrict__ d,
+                                                 float* __restrict__ out) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    float pe[kPE];
+    positional_encoding(d[3 * r], d[3 * r + 1], d[3 * r + 2], pe);
+#pragma unroll
+    for (int j = 0; j < kPE; ++j) out[r * kPE + j] = pe[j];
+}
+
+hipError_t launch_positional_encoding(int64_t R, const float* d, float* pe, hipStream_t s) {
+    if (R <= 0) return hipSuccess;
+    hipLaunchKernelGGL(pe_kernel, dim3((unsigned)((R + 255) / 256)), dim3(256), 0, s, R, d, pe);
+    return hipGetLastError();
+}
+
+// Tiles per launch: one wave per 32-row tile up to 8 waves per CU's worth, then each wave loops.
+static dim3 adj_grid(int64_t n_rows) {
+    const int64_t tiles = (n_rows + 31) / 32;
+    return dim3((unsigned)(tiles < 256 * 8 ? tiles : 256 * 8));
+}
+
+template <class D>
+static hipError_t launch_adj_d(bool backward, const AdjArgs& p, hipStream_t s) {
+    if (backward)
+        hipLaunchKernelGGL((adj_backward_kernel<D>), adj_grid(p.n_rows), dim3(64), 0, s, p);
+    else
+        hipLaunchKernelGGL((adj_forward_kernel<D>), adj_grid(p.n_rows), dim3(64), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_adjoint(int variant, bool backward, const AdjArgs& p, hipStream_t s) {
+    if (p.n_rows == 0) return hipSuccess;
+    switch (variant) {
+        case 0: return launch_adj_d<DecB32>(backward, p, s);
+        case 1: return launch_adj_d<DecB64>(backward, p, s);
+        case 2: return launch_adj_d<DecB64C64>(backward, p, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace diver

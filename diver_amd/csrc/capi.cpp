@@ -90,6 +90,7 @@ struct diver_decoder {
     uint32_t* pack16_f16 = nullptr;
     float* pack_f32 = nullptr;     // FP32 matrix-core A operands (repacked on every update)
     float* pack_pre = nullptr;     // FP32 matrix-core A operands of the pre-layers (repacked on every update)
+    float* pack_adj = nullptr;     // FP32 transposed operands of the decoder adjoint (repacked on every update)
 };
 
 extern "C" {
@@ -315,6 +316,7 @@ int diver_decoder_create(const diver_decoder_desc* desc, const float* params_hos
         if (d->pack16_f16) (void)hipFree(d->pack16_f16);
         if (d->pack_f32) (void)hipFree(d->pack_f32);
         if (d->pack_pre) (void)hipFree(d->pack_pre);
+        if (d->pack_adj) (void)hipFree(d->pack_adj);
         delete d;
         return fail(code, "%s", what);
     };
@@ -363,6 +365,14 @@ int diver_decoder_create(const diver_decoder_desc* desc, const float* params_hos
         if (diver::launch_pack_f32(C, H, desc->n_post, d->W, d->pack_f32, nullptr) != hipSuccess ||
             hipStreamSynchronize(nullptr) != hipSuccess)
             return cleanup(DIVER_EHIP, "decoder FP32 pack failed");
+        if (C % 32 == 0 && H % 32 == 0 && desc->n_post == 1) {
+            const int64_t na = diver::pack_adj_size(C, H, desc->n_post);
+            if (hipMalloc(&d->pack_adj, sizeof(float) * na) != hipSuccess)
+                return cleanup(DIVER_ENOMEM, "decoder pack allocation failed");
+            if (diver::launch_pack_adj(C, H, desc->n_post, d->W, d->pack_adj, nullptr) != hipSuccess ||
+                hipStreamSynchronize(nullptr) != hipSuccess)
+                return cleanup(DIVER_EHIP, "decoder adjoint pack failed");
+        }
     } else if (!desc->pre_identity && desc->channels % 16 == 0 && desc->hidden % 32 == 0) {
         const int64_t nf = diver::pack_pre_f32_size(desc->channels, desc->hidden, desc->n_pre);
         if (hipMalloc(&d->pack_pre, sizeof(float) * nf) != hipSuccess)
@@ -405,9 +415,13 @@ int diver_decoder_update_range(diver_decoder* d, const float* params, int64_t fi
                                                      d->W, d->pack_pre, S(stream)),
                           "decoder_update (FP32 pre-layer pack)");
     if (!d->pack_f32) return st;
-    return hip_status(diver::launch_pack_f32(d->desc.channels, d->desc.hidden, d->desc.n_post, d->W,
-                                             d->pack_f32, S(stream)),
-                      "decoder_update (FP32 pack)");
+    st = hip_status(diver::launch_pack_f32(d->desc.channels, d->desc.hidden, d->desc.n_post, d->W,
+                                           d->pack_f32, S(stream)),
+                    "decoder_update (FP32 pack)");
+    if (st != DIVER_OK || !d->pack_adj) return st;
+    return hip_status(diver::launch_pack_adj(d->desc.channels, d->desc.hidden, d->desc.n_post, d->W,
+                                             d->pack_adj, S(stream)),
+                      "decoder_update (adjoint pack)");
 }
 
 int diver_decoder_destroy(diver_decoder* d) {
@@ -422,6 +436,7 @@ int diver_decoder_destroy(diver_decoder* d) {
     if (d->pack16_f16) (void)hipFree(d->pack16_f16);
     if (d->pack_f32) (void)hipFree(d->pack_f32);
     if (d->pack_pre) (void)hipFree(d->pack_pre);
+    if (d->pack_adj) (void)hipFree(d->pack_adj);
     hipSetDevice(dev);
     delete d;
     return DIVER_OK;
@@ -758,6 +773,81 @@ int diver_render_rows(const diver_grid* grid, const diver_decoder* dec,
     a.d = d;
     return hip_status(diver::launch_rows(dec->variant, a, count, row_offset, rows, S(stream)),
                       "render_rows");
+}
+
+int diver_adjoint_decoder_forward(const diver_decoder* dec, int64_t n_rows, const float* rows,
+                                  const int64_t* ray, const float* d, float* r1, float* r2,
+                                  float* color, float* weight, diver_stream_t stream) {
+    REQUIRE(dec, "null decoder");
+    REQUIRE(dec->pack_adj && dec->pack_f32,
+            "the decoder adjoint needs a decoder without pre-layers, one post layer and channels "
+            "and hidden width that are multiples of 32");
+    REQUIRE(n_rows >= 0, "n_rows < 0");
+    if (n_rows == 0) return DIVER_OK;
+    REQUIRE(rows && ray && d && r1 && r2 && color && weight, "null pointer");
+    diver::AdjArgs p{};
+    p.n_rows = n_rows;
+    p.n_valid = n_rows;
+    p.rows = rows;
+    p.row_stride = 9 + dec->desc.channels;
+    p.ray = ray;
+    p.d = d;
+    p.W = dec->W;
+    p.pack = dec->pack_f32;
+    p.pack_adj = dec->pack_adj;
+    p.r1 = r1;
+    p.r2 = r2;
+    p.color = color;
+    p.weight = weight;
+    return hip_status(diver::launch_adjoint(dec->variant, false, p, S(stream)),
+                      "adjoint_decoder_forward");
+}
+
+int diver_adjoint_decoder_backward(const diver_decoder* dec, int64_t n_rows, int64_t n_valid,
+                                   const float* rows, const int64_t* ray, const float* grad_rgb,
+                                   const float* suffix, const float* sparsity_scale,
+                                   const float* r1, const float* r2, const float* color,
+                                   const float* weight, float* d_feature, float* d_out, float* d_s,
+                                   float* d_r1, float* d_r2, diver_stream_t stream) {
+    REQUIRE(dec, "null decoder");
+    REQUIRE(dec->pack_adj && dec->pack_f32,
+            "the decoder adjoint needs a decoder without pre-layers, one post layer and channels "
+            "and hidden width that are multiples of 32");
+    REQUIRE(n_rows >= 0 && n_valid >= 0 && n_valid <= n_rows, "need 0 <= n_valid <= n_rows");
+    if (n_rows == 0) return DIVER_OK;
+    REQUIRE(rows && ray && grad_rgb && r1 && r2 && color && weight && d_out && d_s && d_r1 && d_r2,
+            "null pointer");
+    REQUIRE(n_valid == 0 || (suffix && d_feature), "null pointer");
+    diver::AdjArgs p{};
+    p.n_rows = n_rows;
+    p.n_valid = n_valid;
+    p.rows = rows;
+    p.row_stride = 9 + dec->desc.channels;
+    p.ray = ray;
+    p.W = dec->W;
+    p.pack = dec->pack_f32;
+    p.pack_adj = dec->pack_adj;
+    p.r1 = const_cast<float*>(r1);
+    p.r2 = const_cast<float*>(r2);
+    p.color = const_cast<float*>(color);
+    p.weight = const_cast<float*>(weight);
+    p.g = grad_rgb;
+    p.suffix = suffix;
+    p.sparsity_scale = sparsity_scale;
+    p.dF = d_feature;
+    p.dout = d_out;
+    p.ds = d_s;
+    p.dr1 = d_r1;
+    p.dr2 = d_r2;
+    return hip_status(diver::launch_adjoint(dec->variant, true, p, S(stream)),
+                      "adjoint_decoder_backward");
+}
+
+int diver_positional_encoding(int64_t n, const float* d, float* pe, diver_stream_t stream) {
+    REQUIRE(n >= 0, "n < 0");
+    if (n == 0) return DIVER_OK;
+    REQUIRE(d && pe, "null pointer");
+    return hip_status(diver::launch_positional_encoding(n, d, pe, S(stream)), "positional_encoding");
 }
 
 int diver_ray_suffix(int64_t n_rays, const int64_t* row_offset, const int32_t* count,

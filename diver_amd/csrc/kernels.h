@@ -104,6 +104,35 @@ hipError_t launch_mlp_eval(int variant, int64_t batch, const float* coord, const
 hipError_t launch_render(int variant, const RenderArgs& a, int prec, hipStream_t s);
 hipError_t launch_alpha_map(int variant, const RenderArgs& a, int pad, float* map,
                             const float* pack_pre, hipStream_t s);
+// Decoder and compositing adjoint over the recompute pass's rows (adjoint_mlp.hip).
+struct AdjArgs {
+    int64_t n_rows;          // rows (a bucket; rows past n_valid are zero)
+    int64_t n_valid;         // composited intervals
+    const float* rows;       // [n_rows][row_stride]: p_in, p_out, s, T_before, alpha, F
+    int row_stride;          // 9 + C
+    const int64_t* ray;      // [n_rows] ray of each row
+    const float* d;          // [R][3] ray directions (forward: the view term)
+    const float* W;          // canonical decoder parameters
+    const float* pack;       // FP32 forward operands (pack_f32)
+    const float* pack_adj;   // FP32 transposed operands (pack_adj)
+    float* r1;               // [n_rows][H] relu(y1)
+    float* r2;               // [n_rows][H] relu(y2)
+    float* color;            // [n_rows][3]
+    float* weight;           // [n_rows] T_before * alpha
+    const float* g;          // [R][3] dL/d(rgb)
+    const float* suffix;     // [n_valid][3] per-ray suffix sums of w c (+ background)
+    const float* sparsity_scale;  // device scalar g_reg / (B M), or null
+    float* dF;               // [n_valid][C]
+    float* dout;             // [n_rows][3]
+    float* ds;               // [n_rows]
+    float* dr1;              // [n_rows][H]
+    float* dr2;              // [n_rows][H]
+};
+int64_t pack_adj_size(int C, int H, int npost);
+hipError_t launch_pack_adj(int C, int H, int npost, const float* W, float* out, hipStream_t s);
+hipError_t launch_adjoint(int variant, bool backward, const AdjArgs& p, hipStream_t s);
+hipError_t launch_positional_encoding(int64_t R, const float* d, float* pe, hipStream_t s);
+
 // FP32 matrix-core A operands of a decoder's pre-layers (the occupancy kernel's density MLP),
 // packed on the device from its canonical parameters W: pack_pre_f32_size floats.
 int64_t pack_pre_f32_size(int C, int H, int npre);

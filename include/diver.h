@@ -300,6 +300,38 @@ int diver_ray_suffix(int64_t n_rays, const int64_t* row_offset, const int32_t* c
                      const float* weight, const float* color, const float* background,
                      float* suffix, diver_stream_t stream);
 
+/* Decoder adjoint, forward half, over n_rows rows of diver_render_rows (decoders without
+ * pre-layers, one post layer, channels and hidden width multiples of 32; the decoder's current
+ * parameters).  ray [n_rows] (int64) names each row's ray, d [R][3] the rays' directions.  Writes,
+ * per row, relu(y1) and relu(y2) of the colour MLP (r1, r2 [n_rows][hidden]), the colour
+ * color [n_rows][3] and the compositing weight weight [n_rows] = T_before * alpha, evaluated as the
+ * fused renderer evaluates them (FP32 operands on the matrix cores).  Together with
+ * diver_adjoint_decoder_backward this replaces the autograd of mlp and render over stored samples
+ * in the reference's training step (model/diver.py:159-207, train.py:102-113). */
+int diver_adjoint_decoder_forward(const diver_decoder* dec, int64_t n_rows, const float* rows,
+                                  const int64_t* ray, const float* d, float* r1, float* r2,
+                                  float* color, float* weight, diver_stream_t stream);
+
+/* Decoder adjoint, backward half.  Rows n_valid .. n_rows - 1 are padding (zero rows).  With
+ * grad_rgb [R][3] = dL/d(rgb), suffix [n_valid][3] from diver_ray_suffix over (weight, color) and
+ * sparsity_scale a device scalar (dL/d(sigma_loss) / (B M), or NULL for no sparsity term), writes
+ *   d_out [n_rows][3]   dL/d(W_o r2 + b_o) = g w c (1 - c)
+ *   d_s [n_rows]        dL/ds = g . (T_after c - suffix) act'(s) (+ sparsity_scale 4s / (1 + 2s^2))
+ *   d_r2, d_r1 [n_rows][hidden]   dL/dy2 = (W_o^T d_out) [y2 > 0], dL/dy1 = (W_2^T d_r2) [y1 > 0]
+ *   d_feature [n_valid][channels] dL/dF = W_c^T d_r1 + d_s w_s
+ * The weight gradients are the reductions d_out^T r2, d_r2^T r1, d_r1^T [F, PE(d)] and d_s^T F over
+ * the rows; d_feature goes to diver_integrate_backward. */
+int diver_adjoint_decoder_backward(const diver_decoder* dec, int64_t n_rows, int64_t n_valid,
+                                   const float* rows, const int64_t* ray, const float* grad_rgb,
+                                   const float* suffix, const float* sparsity_scale,
+                                   const float* r1, const float* r2, const float* color,
+                                   const float* weight, float* d_feature, float* d_out, float* d_s,
+                                   float* d_r1, float* d_r2, diver_stream_t stream);
+
+/* The positional encoding PE(d) of n directions, [n][27]: [d, sin(pi d), cos(pi d), ...,
+ * sin(8 pi d), cos(8 pi d)] (model/mlps.py:12-18), as the fused kernels evaluate it. */
+int diver_positional_encoding(int64_t n, const float* d, float* pe, diver_stream_t stream);
+
 /* 8^3-block occupancy bits of an [n^3] mask (n a multiple of 8): bit b of word b/32 is set
  * when block b = (bz*(n/8) + by)*(n/8) + bx holds an occupied voxel; bits has
  * ceil((n/8)^3 / 32) words.  No counterpart in the reference (an acceleration structure for
