@@ -771,7 +771,7 @@ int diver_render_rows(const diver_grid* grid, const diver_decoder* dec,
     a.n_rays = n_rays;
     a.o = o;
     a.d = d;
-    return hip_status(diver::launch_rows(dec->variant, a, count, row_offset, rows, S(stream)),
+    return hip_status(diver::launch_rows(dec->variant, a, dec->density_ch0, count, row_offset, rows, S(stream)),
                       "render_rows");
 }
 

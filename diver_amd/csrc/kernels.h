@@ -163,8 +163,8 @@ hipError_t launch_brick_bits(const uint8_t* mask, int n, unsigned long long* bri
 hipError_t launch_walk(const RenderArgs& a, int max_n, float* isect, int32_t* num, hipStream_t s);
 hipError_t launch_ray_suffix(int64_t R, const int64_t* off, const int32_t* cnt, const float* w,
                              const float* c, const float* bg, float* out, hipStream_t s);
-hipError_t launch_rows(int variant, const RenderArgs& a, const int32_t* count, const int64_t* off,
-                       float* rows, hipStream_t s);
+hipError_t launch_rows(int variant, const RenderArgs& a, bool dch0, const int32_t* count,
+                       const int64_t* off, float* rows, hipStream_t s);
 hipError_t launch_camera_rays(const CameraDesc& cam, float* o, float* d, hipStream_t s);
 
 }  // namespace diver
