@@ -51,7 +51,7 @@ constexpr int kBlocksInLds = 0;    // 8^3-block bitmap copied to LDS (1) or read
 // A wave still walking after this many phase-A iterations gets the highest issue priority: the
 // long central tiles finish sooner while short waves fill the gaps (2 % on batched frames).
 constexpr int kLongWalk = 256;
-constexpr int kCornerRound = 4;   // corners whose feature loads are in flight together (16-bit phase B)
+constexpr int kCornerRound = 4;   // 32-channel corner rows whose loads are in flight together (16-bit phase B)
 
 // The per-ray view terms of the BF16 decoder are kept as packed BF16 pairs, the decoder's operand
 // precision (half the LDS of FP32 terms); the FP32 and FP16 decoders keep them in FP32.
@@ -168,36 +168,47 @@ __device__ __forceinline__ void shade_batch16(const RenderArgs& a, const float* 
             float w[8];
             uint32_t off[8];
             segment_corners<C, uint32_t>(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, n, w, off);
+            // each channel is the FP32 sum over corners j = 0..7 in that order; the loads go in
+            // rounds of kCornerRound / KS0 corners (every k step of a corner together), eight
+            // 16-byte loads in flight per round whatever the width
+            constexpr int kR = kCornerRound / KS0;
+            static_assert(kR >= 1 && kCornerRound % KS0 == 0, "corner rounds");
+            float s[KS0][8];
 #pragma unroll
-            for (int kk = 0; kk < KS0; ++kk) {
-                // each channel is the FP32 sum over corners j = 0..7 in that order; the corners'
-                // loads go in kCornerRound-corner rounds to bound the registers in flight
-                float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f, s4 = 0.f, s5 = 0.f, s6 = 0.f, s7 = 0.f;
+            for (int kk = 0; kk < KS0; ++kk)
 #pragma unroll
-                for (int j0 = 0; j0 < 8; j0 += kCornerRound) {
-                    float4 q0[kCornerRound], q1[kCornerRound];
+                for (int c = 0; c < 8; ++c) s[kk][c] = 0.f;
 #pragma unroll
-                    for (int j = 0; j < kCornerRound; ++j) {
+            for (int j0 = 0; j0 < 8; j0 += kR) {
+                float4 q0[kR][KS0], q1[kR][KS0];
+#pragma unroll
+                for (int j = 0; j < kR; ++j)
+#pragma unroll
+                    for (int kk = 0; kk < KS0; ++kk) {
                         const float* src = feature_at<B32>(a.voxels, off[j0 + j] + 32u * kk + 4u * gp);
-                        q0[j] = *reinterpret_cast<const float4*>(src);
-                        q1[j] = *reinterpret_cast<const float4*>(src + 16);
+                        q0[j][kk] = *reinterpret_cast<const float4*>(src);
+                        q1[j][kk] = *reinterpret_cast<const float4*>(src + 16);
                     }
 #pragma unroll
-                    for (int j = 0; j < kCornerRound; ++j) {
-                        s0 += w[j0 + j] * q0[j].x;
-                        s1 += w[j0 + j] * q0[j].y;
-                        s2 += w[j0 + j] * q0[j].z;
-                        s3 += w[j0 + j] * q0[j].w;
-                        s4 += w[j0 + j] * q1[j].x;
-                        s5 += w[j0 + j] * q1[j].y;
-                        s6 += w[j0 + j] * q1[j].z;
-                        s7 += w[j0 + j] * q1[j].w;
+                for (int j = 0; j < kR; ++j)
+#pragma unroll
+                    for (int kk = 0; kk < KS0; ++kk) {
+                        const float wj = w[j0 + j];
+                        s[kk][0] += wj * q0[j][kk].x;
+                        s[kk][1] += wj * q0[j][kk].y;
+                        s[kk][2] += wj * q0[j][kk].z;
+                        s[kk][3] += wj * q0[j][kk].w;
+                        s[kk][4] += wj * q1[j][kk].x;
+                        s[kk][5] += wj * q1[j][kk].y;
+                        s[kk][6] += wj * q1[j][kk].z;
+                        s[kk][7] += wj * q1[j][kk].w;
                     }
-                    __builtin_amdgcn_sched_barrier(0);
-                }
-                b[kk] = u32x4_t{pack2<PREC>(s0, s1), pack2<PREC>(s2, s3), pack2<PREC>(s4, s5),
-                                pack2<PREC>(s6, s7)};
+                __builtin_amdgcn_sched_barrier(0);
             }
+#pragma unroll
+            for (int kk = 0; kk < KS0; ++kk)
+                b[kk] = u32x4_t{pack2<PREC>(s[kk][0], s[kk][1]), pack2<PREC>(s[kk][2], s[kk][3]),
+                                pack2<PREC>(s[kk][4], s[kk][5]), pack2<PREC>(s[kk][6], s[kk][7])};
         } else {
 #pragma unroll
             for (int kk = 0; kk < KS0; ++kk) b[kk] = u32x4_t{0u, 0u, 0u, 0u};  // +0 in both halves
@@ -353,14 +364,20 @@ __device__ __forceinline__ void phase_a(TileLane& L, const BatchArgs& p, const u
     }
 }
 
-// Register budget: 3 waves per SIMD for hidden width 32 (the BF16 kernel compiles to at most 128
-// VGPRs, which allows 4).  Wider decoders keep 64 x H view terms in LDS (16 KB in FP32 at H = 64,
-// 8 waves per CU), so they take the 2-wave budget.
-template <class D>
-constexpr int tile_waves_per_simd() { return D::H > 32 ? 2 : 3; }
+// Register budget: 3 waves per SIMD (168 VGPRs) for hidden width 32 (the BF16 kernel compiles to
+// at most 128 VGPRs, which allows 4).  Hidden width 64 takes the 2-wave budget (the 32-channel
+// kernels compile to about 130 VGPRs and run 3 waves anyway; the FP32 and linear-density forms
+// need more than 168), except the 16-bit channel-0 kernels at 64 channels: held to 168 VGPRs they
+// park one value (the ray's output slot) in scratch, stored once before the walk and loaded once
+// after it, and shade a 256^3 x 64 scene 13 % faster than at 2 waves.
+template <class D, int PREC, bool DCH0>
+constexpr int tile_waves_per_simd() {
+    if (D::H <= 32) return 3;
+    return D::C > 32 && PREC != kPrecFP32 && DCH0 ? 3 : 2;
+}
 
 template <class D, int PREC, bool RT, bool DCH0, bool B32>
-__global__ void __launch_bounds__(64 * kWavesPerGroup, tile_waves_per_simd<D>()) render_tile_kernel(BatchArgs p) {
+__global__ void __launch_bounds__(64 * kWavesPerGroup, (tile_waves_per_simd<D, PREC, DCH0>())) render_tile_kernel(BatchArgs p) {
     static_assert(D::NPRE == 0, "the tile kernel needs density from the integrated feature");
     constexpr int C = D::C, H = D::H, Z = D::Z;
     const RenderArgs& a = p.g;
