@@ -18,7 +18,7 @@ EXPORTS = [
     'diver_aabb_intersect', 'diver_ray_march', 'diver_integrate', 'diver_integrate_backward',
     'diver_composite', 'diver_composite_backward', 'diver_decoder_create', 'diver_decoder_destroy',
     'diver_decoder_param_count', 'diver_mlp_eval', 'diver_render_rays', 'diver_render_camera',
-    'diver_camera_rays', 'diver_block_bits', 'diver_walk_rays', 'diver_render_rows',
+    'diver_camera_rays', 'diver_block_bits', 'diver_tail_bits', 'diver_walk_rays', 'diver_render_rows',
     'diver_decoder_update', 'diver_brick_bits', 'diver_render_batch', 'diver_alpha_map',
     'diver_corner_keys', 'diver_integrate_table', 'diver_integrate_table_backward',
     'diver_adam_step', 'diver_decoder_update_range', 'diver_ray_suffix',
@@ -57,7 +57,7 @@ class Grid(ctypes.Structure):
                 ('octrees', ctypes.c_void_p), ('blocks', ctypes.c_void_p),
                 ('bricks', ctypes.c_void_p), ('n', ctypes.c_int),
                 ('channels', ctypes.c_int), ('xyzmin', ctypes.c_float),
-                ('voxel_size', ctypes.c_float)]
+                ('voxel_size', ctypes.c_float), ('tail', ctypes.c_void_p)]
 
 
 class RenderOpts(ctypes.Structure):
@@ -114,6 +114,7 @@ def _declare(L):
                                 ctypes.POINTER(ctypes.c_float), i32, i32, fl, vp, vp, vp],
         'diver_camera_rays': [ctypes.POINTER(ctypes.c_float), i32, i32, fl, vp, vp, vp],
         'diver_block_bits': [vp, i32, vp, vp],
+        'diver_tail_bits': [vp, i32, vp, vp],
         'diver_brick_bits': [vp, i32, vp, vp],
         'diver_walk_rays': [ctypes.POINTER(Grid), i32, i64, vp, vp, i32, vp, vp, vp],
         'diver_render_rows': [ctypes.POINTER(Grid), vp, ctypes.POINTER(RenderOpts), i64, vp, vp,

@@ -175,11 +175,11 @@ PRECISIONS = {'fp32': 0, 'bf16': 1, 'fp16': 2}
 TRAVERSALS = {'offline': 0, 'realtime': 1}
 
 
-def grid_struct(voxels, mask, octrees, xyzmin, voxel_size, blocks=None, bricks=None):
+def grid_struct(voxels, mask, octrees, xyzmin, voxel_size, blocks=None, bricks=None, tail=None):
     n = mask.shape[0]
     p = lambda t: t.data_ptr() if t is not None else None
     return _lib.Grid(voxels.data_ptr(), mask.data_ptr(), p(octrees), p(blocks), p(bricks), int(n),
-                     int(voxels.shape[-1]), float(xyzmin), float(voxel_size))
+                     int(voxels.shape[-1]), float(xyzmin), float(voxel_size), p(tail))
 
 
 def brick_bits(mask):
@@ -190,6 +190,19 @@ def brick_bits(mask):
     bricks = torch.empty((n // 4) ** 3, dtype=torch.int64, device=mask.device)
     _lib.call('diver_brick_bits', mask.data_ptr(), int(n), bricks.data_ptr(), _s(mask))
     return bricks
+
+
+def tail_bits(blocks, n):
+    """Tail bits of a mask from its 8^3-block bits (n a multiple of 8): the block bits dilated by
+    one block (int32 words, same numbering).  A grid carrying them ends each walk once nothing
+    occupied can lie ahead (see include/diver.h)."""
+    _dev(blocks)
+    if blocks.dtype != torch.int32:
+        raise TypeError('blocks must be int32 words')
+    nb = int(n) // 8
+    tail = torch.empty((nb ** 3 + 31) // 32, dtype=torch.int32, device=blocks.device)
+    _lib.call('diver_tail_bits', blocks.data_ptr(), int(n), tail.data_ptr(), _s(blocks))
+    return tail
 
 
 def block_bits(mask):
@@ -206,7 +219,8 @@ def block_bits(mask):
 
 def _grid_of(grid):
     return grid_struct(grid.voxels, grid.mask, grid.octrees, grid.xyzmin, grid.voxel_size,
-                       getattr(grid, 'blocks', None), getattr(grid, 'bricks', None))
+                       getattr(grid, 'blocks', None), getattr(grid, 'bricks', None),
+                       getattr(grid, 'tail', None))
 
 
 def render_opts(precision='fp32', traversal='realtime', t_stop=1e-4, alpha_skip=0.0):

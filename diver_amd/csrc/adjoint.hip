@@ -5,10 +5,11 @@
 // same compensated transmittance, so the same values) and writes one row per composited
 // interval at the ray's row offset:
 //   [p_in (3), p_out (3), s = w_s.F + b_s, T_before, alpha, F (C)]
-// For decoders whose density is softplus(F[0]) the walk evaluates channel 0 only, as the forward
-// does, and a second kernel gathers every row's F in parallel.  The decoder and compositing
-// adjoints then run over the rows (adjoint_mlp.hip), and the grid gradient is scattered by the
-// segment-integral adjoint.
+// With the block/brick bits the walk writes each row's segment only; a second kernel gathers every
+// row's F (and s, sigma, alpha) in parallel, and a third runs each ray's rows in order for
+// T_before, so no feature load sits on the walk's serial chain (the walk's length is the forward's count, not
+// a transmittance test).  The decoder and compositing adjoints then run over the rows
+// (adjoint_mlp.hip), and the grid gradient is scattered by the segment-integral adjoint.
 #include "common.h"
 #include "decoder.h"
 #include "kernels.h"
@@ -16,20 +17,15 @@
 
 namespace diver {
 
-// The walk of one lane's ray, writing the row header [p_in, p_out, s, T_before, alpha] of each
-// composited interval (and F unless DCH0, whose density needs channel 0 only: the features are
-// then gathered afterwards by rows_features_kernel, in parallel over all rows instead of on the
-// walk's serial chain).  AF: every live ray of the wave takes the reciprocal-division step (no
-// step bound needed, as in the fused renderer's walk).
-template <class D, bool RT, bool DCH0, bool AF>
+// The walk of one lane's ray, writing the segment [p_in, p_out] of each composited interval (the
+// first `todo` occupied voxels).  AF: every live ray of the wave takes the reciprocal-division
+// step (no step bound needed, as in the fused renderer's walk).
+template <class D, bool RT, bool AF>
 __device__ __forceinline__ void rows_walk(RayWalk& ray, const RenderArgs& a, int todo, float* out) {
-    constexpr int C = D::C;
-    constexpr int RW = 9 + C;
+    constexpr int RW = 9 + D::C;
     const int n = a.n;
     const float nf = (float)n;
     OccCache occ;
-    Kahan S;
-    float T = 1.0f;
     int k = 0;
     if (todo <= 0) ray.alive = false;
     while (wave_any(ray.alive)) {
@@ -57,34 +53,9 @@ __device__ __forceinline__ void rows_walk(RayWalk& ray, const RenderArgs& a, int
         ray.ly = ny;
         ray.lz = nz;
         if (!occ.lookup(a.mask, a.blocks, a.bricks, n, vx, vy, vz)) continue;
-        float w[8];
-        int64_t off[8];
-        segment_corners<C>(px, py, pz, nx, ny, nz, n, w, off);
         float* o = out + (int64_t)k * RW;
-        float s;
-        if constexpr (DCH0) {
-            // the fused forward's channel-0 integral: the same corners, order and roundings
-            s = 0.0f;
-#pragma unroll
-            for (int j = 0; j < 8; ++j) s += w[j] * a.voxels[off[j]];
-        } else {
-            float f[C];
-            gather_integral<C>(a.voxels, w, off, f);
-            s = a.W[D::bs_off];
-#pragma unroll
-            for (int i = 0; i < C; ++i) s += a.W[D::ws_off + i] * f[i];
-#pragma unroll
-            for (int i = 0; i < C; ++i) o[9 + i] = f[i];
-        }
-        float sigma, alpha;
-        density_alpha<D>(s, sigma, alpha);
         o[0] = px; o[1] = py; o[2] = pz;
         o[3] = nx; o[4] = ny; o[5] = nz;
-        o[6] = s;
-        o[7] = T;
-        o[8] = alpha;
-        S.add(sigma);
-        T = __expf(-S.s);
         if (++k == todo) ray.alive = false;
     }
 }
@@ -92,7 +63,7 @@ __device__ __forceinline__ void rows_walk(RayWalk& ray, const RenderArgs& a, int
 // One wave per workgroup; lanes [0, rpw) take consecutive rays (rays_per_wave: small batches
 // spread over more waves).  The walk reads occupancy through the block/brick bits (the forward's
 // lookup); grids without them take rows_kernel_mask.
-template <class D, bool RT, bool DCH0>
+template <class D, bool RT>
 __global__ void __launch_bounds__(64) rows_kernel(RenderArgs a, const int32_t* __restrict__ count,
                                                   const int64_t* __restrict__ row_off,
                                                   float* __restrict__ rows, int rpw) {
@@ -113,9 +84,9 @@ __global__ void __launch_bounds__(64) rows_kernel(RenderArgs a, const int32_t* _
     const int todo = valid ? count[rr] : 0;
     float* out = rows + row_off[rr] * RW;
     if (!wave_any(ray.alive && !ray.fast))
-        rows_walk<D, RT, DCH0, true>(ray, a, todo, out);
+        rows_walk<D, RT, true>(ray, a, todo, out);
     else
-        rows_walk<D, RT, DCH0, false>(ray, a, todo, out);
+        rows_walk<D, RT, false>(ray, a, todo, out);
 }
 
 // The same over the byte mask (grids without block/brick bits).
@@ -168,13 +139,17 @@ __global__ void __launch_bounds__(64) rows_kernel_mask(RenderArgs a, const int32
 
 // F of each row from its segment (gather_integral's corner weights, order and roundings): 8 lanes
 // per row, 4 channels per lane per pass, so a row's 8 corner loads of one channel quad are in
-// flight together and neighbouring lanes read neighbouring 16-byte pieces of a corner row.  The
-// row count K = row_off[R - 1] + count[R - 1] is read on the device; the grid strides over it.
-template <int C>
+// flight together and neighbouring lanes read neighbouring 16-byte pieces of a corner row.  For
+// DCH0 the lane holding channel 0 also writes the row's s = F[0] (the fused forward's channel-0
+// sum: the same corners, order and roundings), sigma (in the T_before slot, until
+// rows_transmittance_kernel replaces it) and alpha.  The row count K = row_off[R - 1] +
+// count[R - 1] is read on the device; the grid strides over it.
+template <class D, bool DCH0>
 __global__ void __launch_bounds__(256) rows_features_kernel(int64_t R, const int64_t* __restrict__ row_off,
                                                             const int32_t* __restrict__ count, int n,
                                                             const float* __restrict__ voxels,
                                                             float* __restrict__ rows) {
+    constexpr int C = D::C;
     constexpr int RW = 9 + C;
     const int64_t K = row_off[R - 1] + count[R - 1];
     const int sub = (int)(threadIdx.x & 7);
@@ -201,6 +176,61 @@ __global__ void __launch_bounds__(256) rows_features_kernel(int64_t R, const int
             o[9 + 4 * c4 + 1] = s1;
             o[9 + 4 * c4 + 2] = s2;
             o[9 + 4 * c4 + 3] = s3;
+            if (DCH0 && c4 == 0) {
+                float sigma, alpha;
+                density_alpha<D>(s0, sigma, alpha);
+                o[6] = s0;
+                o[7] = sigma;
+                o[8] = alpha;
+            }
+        }
+    }
+}
+
+// T_before of each row, one lane per ray over its rows in walk order with the forward's
+// compensated sum of sigma, so every value equals what the forward composited.  DCH0: s, sigma
+// and alpha are in the rows already (sigma in the T_before slot), read 8 rows ahead of the sum;
+// otherwise s = w_s.F + b_s, sigma and alpha are worked out here.
+template <class D, bool DCH0>
+__global__ void __launch_bounds__(256) rows_transmittance_kernel(int64_t R,
+                                                                 const int64_t* __restrict__ row_off,
+                                                                 const int32_t* __restrict__ count,
+                                                                 const float* __restrict__ W,
+                                                                 float* __restrict__ rows) {
+    constexpr int C = D::C;
+    constexpr int RW = 9 + C;
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    float* o = rows + row_off[r] * RW;
+    const int m = count[r];
+    Kahan S;
+    float T = 1.0f;
+    if constexpr (DCH0) {
+        for (int k0 = 0; k0 < m; k0 += 8) {
+            float sg[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) sg[j] = k0 + j < m ? o[(int64_t)(k0 + j) * RW + 7] : 0.0f;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                if (k0 + j < m) {
+                    o[(int64_t)(k0 + j) * RW + 7] = T;
+                    S.add(sg[j]);
+                    T = __expf(-S.s);
+                }
+            }
+        }
+    } else {
+        for (int k = 0; k < m; ++k, o += RW) {
+            float sv = W[D::bs_off];
+#pragma unroll
+            for (int i = 0; i < C; ++i) sv += W[D::ws_off + i] * o[9 + i];
+            float sigma, alpha;
+            density_alpha<D>(sv, sigma, alpha);
+            o[6] = sv;
+            o[7] = T;
+            o[8] = alpha;
+            S.add(sigma);
+            T = __expf(-S.s);
         }
     }
 }
@@ -254,19 +284,27 @@ static hipError_t launch_rows_d(const RenderArgs& a, bool dch0, const int32_t* c
             hipLaunchKernelGGL((rows_kernel_mask<D, false>), grid, block, 0, s, a, count, off, rows, rpw);
         return hipGetLastError();
     }
-    if (rt && dch0)
-        hipLaunchKernelGGL((rows_kernel<D, true, true>), grid, block, 0, s, a, count, off, rows, rpw);
-    else if (rt)
-        hipLaunchKernelGGL((rows_kernel<D, true, false>), grid, block, 0, s, a, count, off, rows, rpw);
-    else if (dch0)
-        hipLaunchKernelGGL((rows_kernel<D, false, true>), grid, block, 0, s, a, count, off, rows, rpw);
+    if (rt)
+        hipLaunchKernelGGL((rows_kernel<D, true>), grid, block, 0, s, a, count, off, rows, rpw);
     else
-        hipLaunchKernelGGL((rows_kernel<D, false, false>), grid, block, 0, s, a, count, off, rows, rpw);
+        hipLaunchKernelGGL((rows_kernel<D, false>), grid, block, 0, s, a, count, off, rows, rpw);
     hipError_t e = hipGetLastError();
-    if (e != hipSuccess || !dch0) return e;
+    if (e != hipSuccess) return e;
     // 2048 x 256 lanes: 65,536 rows per pass over the grid, enough to fill the GPU
-    hipLaunchKernelGGL((rows_features_kernel<D::C>), dim3(2048), dim3(256), 0, s, a.n_rays, off, count,
-                       a.n, a.voxels, rows);
+    const dim3 rgrid((unsigned)((a.n_rays + 255) / 256));
+    if (dch0) {
+        hipLaunchKernelGGL((rows_features_kernel<D, true>), dim3(2048), dim3(256), 0, s, a.n_rays, off,
+                           count, a.n, a.voxels, rows);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        hipLaunchKernelGGL((rows_transmittance_kernel<D, true>), rgrid, dim3(256), 0, s, a.n_rays, off,
+                           count, a.W, rows);
+    } else {
+        hipLaunchKernelGGL((rows_features_kernel<D, false>), dim3(2048), dim3(256), 0, s, a.n_rays, off,
+                           count, a.n, a.voxels, rows);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        hipLaunchKernelGGL((rows_transmittance_kernel<D, false>), rgrid, dim3(256), 0, s, a.n_rays, off,
+                           count, a.W, rows);
+    }
     return hipGetLastError();
 }
 

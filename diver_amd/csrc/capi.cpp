@@ -503,6 +503,8 @@ int diver_walk_rays(const diver_grid* g, int traversal, int64_t n_rays, const fl
     a.octrees = g->octrees;
     a.blocks = g->blocks;
     a.bricks = reinterpret_cast<const unsigned long long*>(g->bricks);
+    if (g->tail) REQUIRE(g->n % 8 == 0, "tail bits need n to be a multiple of 8");
+    a.tail = g->tail;
     a.traversal = traversal;
     a.n = g->n;
     a.xyzmin = g->xyzmin;
@@ -511,6 +513,12 @@ int diver_walk_rays(const diver_grid* g, int traversal, int64_t n_rays, const fl
     a.o = o;
     a.d = d;
     return hip_status(diver::launch_walk(a, max_n, isect, num, S(stream)), "walk_rays");
+}
+
+int diver_tail_bits(const uint32_t* blocks, int n, uint32_t* tail, diver_stream_t stream) {
+    REQUIRE(blocks && tail, "null pointer");
+    REQUIRE(n >= 8 && n % 8 == 0 && n <= 4096, "tail bits need n a multiple of 8 in [8, 4096]");
+    return hip_status(diver::launch_tail_bits(blocks, n, tail, S(stream)), "tail_bits");
 }
 
 int diver_brick_bits(const uint8_t* mask, int n, uint64_t* bricks, diver_stream_t stream) {
@@ -600,6 +608,8 @@ static int render_common(const diver_grid* g, const diver_decoder* dec,
     a.bricks = reinterpret_cast<const unsigned long long*>(g->bricks);
     if (g->blocks || g->bricks)
         REQUIRE(g->n % 8 == 0, "block and brick bits need n to be a multiple of 8");
+    if (g->tail) REQUIRE(g->n % 8 == 0, "tail bits need n to be a multiple of 8");
+    a.tail = g->tail;
     a.traversal = opts->traversal;
     a.n = g->n;
     a.channels = g->channels;

@@ -176,6 +176,14 @@ __device__ __forceinline__ float log1p_unit(float e) {
     return fmaf(t * z2, p, t);
 }
 
+// The sparsity term log(1 + s^2 / 0.5) of one interval (train.py:84-85, where torch takes it in
+// FP32 as log(1 + x)): log1p_unit for x = 2 s^2 <= 1, and log(1 + x) above, where rounding 1 + x
+// costs at most 2^-24 / log 2 relative.  About 20 operations against ~110 for the library log1pf.
+__device__ __forceinline__ float sparsity_term(float s) {
+    const float x = 2.0f * s * s;
+    return x <= 1.0f ? log1p_unit(x) : __logf(1.0f + x);
+}
+
 // e = exp(-|s|): alpha = 1 / (1 + e) for s > 0 and e / (1 + e) otherwise, which shares the
 // exponential with softplus.
 template <class D>

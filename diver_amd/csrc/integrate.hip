@@ -3,7 +3,8 @@
 //
 // Layout: voxels [N+1][N+1][N+1][C] fp32, z-major, channel fastest; coords [K][3] (x, y, z) in grid
 // units.  A row is handled by C/4 lanes, each owning 4 channels (one 16-byte load per corner),
-// so the 8 corner reads of a row are whole 128-byte lines at C=32.
+// so the 8 corner reads of a row are whole 128-byte lines at C=32; the adjoint's atomic adds go
+// one lane per channel, whole rows per instruction.
 #include "common.h"
 #include "kernels.h"
 
@@ -88,35 +89,35 @@ __global__ void __launch_bounds__(256) integrate_kernel(const float* __restrict_
     }
 }
 
-template <int LANES>
+// The adjoint scatters with one lane per channel (LPR lanes per row: C rounded up to a power of
+// two, at most 64; channels past 64 loop), so each atomic instruction of a wave adds whole corner
+// rows -- 64 / LPR rows of C contiguous floats -- where a float4-per-lane layout would touch a
+// 16-byte piece of eight rows per instruction (four times the cache-line requests for the same
+// adds; the atomics bound this kernel).
+template <int LPR>
 __global__ void __launch_bounds__(256) integrate_backward_kernel(int n1, int C, int64_t K,
                                                                  const float* __restrict__ ci,
                                                                  const float* __restrict__ co,
                                                                  const float* __restrict__ gout,
                                                                  float* __restrict__ gvox) {
-    int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t row = gt / LANES;
-    int sub = (int)(gt % LANES);
+    const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t row = gt / LPR;
+    const int sub = (int)(gt % LPR);
     if (row >= K) return;
-    CornerWeights cw = corner_weights(ci[3 * row], ci[3 * row + 1], ci[3 * row + 2], co[3 * row],
-                                      co[3 * row + 1], co[3 * row + 2]);
+    const CornerWeights cw = corner_weights(ci[3 * row], ci[3 * row + 1], ci[3 * row + 2],
+                                            co[3 * row], co[3 * row + 1], co[3 * row + 2]);
     const int64_t sz = (int64_t)n1 * n1 * C, sy = (int64_t)n1 * C, sx = C;
     // the upper corner is clamped to the grid as in integrator.py:79; the lower one only guards
     // malformed coordinates (valid ones lie in [0, N])
-    int xs[2] = {min(max(cw.cx, 0), n1 - 1), min(max(cw.cx + 1, 0), n1 - 1)};
-    int ys[2] = {min(max(cw.cy, 0), n1 - 1), min(max(cw.cy + 1, 0), n1 - 1)};
-    int zs[2] = {min(max(cw.cz, 0), n1 - 1), min(max(cw.cz + 1, 0), n1 - 1)};
-    for (int c4 = sub; c4 < C / 4; c4 += LANES) {
-        float4 g = *reinterpret_cast<const float4*>(gout + row * C + 4 * c4);
+    const int xs[2] = {min(max(cw.cx, 0), n1 - 1), min(max(cw.cx + 1, 0), n1 - 1)};
+    const int ys[2] = {min(max(cw.cy, 0), n1 - 1), min(max(cw.cy + 1, 0), n1 - 1)};
+    const int zs[2] = {min(max(cw.cz, 0), n1 - 1), min(max(cw.cz + 1, 0), n1 - 1)};
+    for (int c = sub; c < C; c += LPR) {
+        const float g = gout[row * C + c];
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-            int64_t base = zs[j >> 2] * sz + ys[(j >> 1) & 1] * sy + (int64_t)xs[j & 1] * sx;
-            float* dst = gvox + base + 4 * c4;
-            float w = cw.w[j];
-            unsafeAtomicAdd(dst + 0, w * g.x);
-            unsafeAtomicAdd(dst + 1, w * g.y);
-            unsafeAtomicAdd(dst + 2, w * g.z);
-            unsafeAtomicAdd(dst + 3, w * g.w);
+            const int64_t base = zs[j >> 2] * sz + ys[(j >> 1) & 1] * sy + (int64_t)xs[j & 1] * sx;
+            unsafeAtomicAdd(gvox + base + c, cw.w[j] * g);
         }
     }
 }
@@ -171,27 +172,23 @@ __global__ void __launch_bounds__(256) integrate_table_kernel(const float* __res
     }
 }
 
-template <int LANES>
+// integrate_backward_kernel with corner j of row r scattered into table row crow[8 r + j] (one
+// lane per channel, as there).
+template <int LPR>
 __global__ void __launch_bounds__(256) integrate_table_backward_kernel(
     int C, int64_t K, const int64_t* __restrict__ crow, const float* __restrict__ ci,
     const float* __restrict__ co, const float* __restrict__ gout, float* __restrict__ gtable) {
     const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t row = gt / LANES;
-    const int sub = (int)(gt % LANES);
+    const int64_t row = gt / LPR;
+    const int sub = (int)(gt % LPR);
     if (row >= K) return;
     const CornerWeights cw = corner_weights(ci[3 * row], ci[3 * row + 1], ci[3 * row + 2],
                                             co[3 * row], co[3 * row + 1], co[3 * row + 2]);
-    for (int c4 = sub; c4 < C / 4; c4 += LANES) {
-        const float4 g = *reinterpret_cast<const float4*>(gout + row * C + 4 * c4);
+    for (int c = sub; c < C; c += LPR) {
+        const float g = gout[row * C + c];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            float* dst = gtable + crow[8 * row + j] * C + 4 * c4;
-            const float w = cw.w[j];
-            unsafeAtomicAdd(dst + 0, w * g.x);
-            unsafeAtomicAdd(dst + 1, w * g.y);
-            unsafeAtomicAdd(dst + 2, w * g.z);
-            unsafeAtomicAdd(dst + 3, w * g.w);
-        }
+        for (int j = 0; j < 8; ++j)
+            unsafeAtomicAdd(gtable + crow[8 * row + j] * C + c, cw.w[j] * g);
     }
 }
 
@@ -233,11 +230,35 @@ hipError_t launch_integrate_table(const float* table, int C, int64_t K, const in
     return hipGetLastError();
 }
 
+// Lanes per row of the scatters: C rounded up to a power of two, at most 64 (C = 12 gives 16
+// lanes, 4 of them idle; C = 96 gives 64 lanes, some taking two channels).
+static int scatter_lanes_for(int C) {
+    int lanes = 1;
+    while (lanes < C && lanes < 64) lanes *= 2;
+    return lanes;
+}
+
+#define DIVER_SCATTER_LAUNCH(KERNEL, ...)                                                      \
+    do {                                                                                       \
+        const int lanes = scatter_lanes_for(C);                                                \
+        const int64_t threads = K * lanes;                                                     \
+        const dim3 grid((unsigned)((threads + 255) / 256));                                    \
+        switch (lanes) {                                                                       \
+            case 1: hipLaunchKernelGGL(KERNEL<1>, grid, dim3(256), 0, s, __VA_ARGS__); break;  \
+            case 2: hipLaunchKernelGGL(KERNEL<2>, grid, dim3(256), 0, s, __VA_ARGS__); break;  \
+            case 4: hipLaunchKernelGGL(KERNEL<4>, grid, dim3(256), 0, s, __VA_ARGS__); break;  \
+            case 8: hipLaunchKernelGGL(KERNEL<8>, grid, dim3(256), 0, s, __VA_ARGS__); break;  \
+            case 16: hipLaunchKernelGGL(KERNEL<16>, grid, dim3(256), 0, s, __VA_ARGS__); break; \
+            case 32: hipLaunchKernelGGL(KERNEL<32>, grid, dim3(256), 0, s, __VA_ARGS__); break; \
+            default: hipLaunchKernelGGL(KERNEL<64>, grid, dim3(256), 0, s, __VA_ARGS__); break; \
+        }                                                                                      \
+    } while (0)
+
 hipError_t launch_integrate_table_backward(int C, int64_t K, const int64_t* crow, const float* ci,
                                            const float* co, const float* gout, float* gtable,
                                            hipStream_t s) {
     if (K == 0) return hipSuccess;
-    DIVER_LANES_LAUNCH(integrate_table_backward_kernel, C, K, crow, ci, co, gout, gtable);
+    DIVER_SCATTER_LAUNCH(integrate_table_backward_kernel, C, K, crow, ci, co, gout, gtable);
     return hipGetLastError();
 }
 
@@ -251,10 +272,11 @@ hipError_t launch_integrate(const float* voxels, int n1, int C, int64_t K, const
 hipError_t launch_integrate_backward(int n1, int C, int64_t K, const float* ci, const float* co,
                                      const float* gout, float* gvox, hipStream_t s) {
     if (K == 0) return hipSuccess;
-    DIVER_LANES_LAUNCH(integrate_backward_kernel, n1, C, K, ci, co, gout, gvox);
+    DIVER_SCATTER_LAUNCH(integrate_backward_kernel, n1, C, K, ci, co, gout, gvox);
     return hipGetLastError();
 }
 
 #undef DIVER_LANES_LAUNCH
+#undef DIVER_SCATTER_LAUNCH
 
 }  // namespace diver

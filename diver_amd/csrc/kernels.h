@@ -18,6 +18,7 @@ struct RenderArgs {
     const uint8_t* octrees;  // occupancy pyramid (block sizes 64, 16, 4), real-time traversal
     const uint32_t* blocks;  // 8^3-block occupancy bits (optional)
     const unsigned long long* bricks;  // bit-packed mask, 64 bits per 4^3 brick (optional)
+    const uint32_t* tail;  // tail bits: 16^3 cells with an occupied voxel around them (optional)
     int traversal;         // 0: masked_intersect walk (exit > N); 1: real-time walk
     int n;
     int channels;
@@ -161,6 +162,7 @@ hipError_t launch_adam_rows(float* p, float* g, float* m, float* v, int64_t n, c
 hipError_t launch_brick_bits(const uint8_t* mask, int n, unsigned long long* bricks,
                              hipStream_t s);
 hipError_t launch_walk(const RenderArgs& a, int max_n, float* isect, int32_t* num, hipStream_t s);
+hipError_t launch_tail_bits(const uint32_t* blocks, int n, uint32_t* tail, hipStream_t s);
 hipError_t launch_ray_suffix(int64_t R, const int64_t* off, const int32_t* cnt, const float* w,
                              const float* c, const float* bg, float* out, hipStream_t s);
 hipError_t launch_rows(int variant, const RenderArgs& a, bool dch0, const int32_t* count,

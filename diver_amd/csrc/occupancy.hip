@@ -5,7 +5,10 @@
 //   - block_bits_kernel: one bit per 8^3 block (any voxel occupied), kept in LDS by the renderer;
 //   - brick_bits_kernel: the mask bit-packed per 4^3 brick (one 64-bit word per brick), read by
 //     the renderer's per-lane brick cache.
-// Neither structure changes which voxels are visited: they only replace byte-mask reads.
+//   - tail_bits_kernel: the block bits dilated by one block, from which each walk bounds where
+//     it can end.
+// None of them changes which occupied voxels are visited: the block and brick bits replace
+// byte-mask reads, and the tail bits only end a walk where nothing occupied remains ahead.
 #include "common.h"
 #include "kernels.h"
 #include "raystate.h"
@@ -32,11 +35,13 @@ __global__ void __launch_bounds__(256) walk_kernel(RenderArgs a, int max_n, floa
     const float nf = (float)n;
     OccCache occ;
     int count = 0;
+    const float tail = tail_limit(a.tail, n, ray);
     while (ray.alive && ray.steps < ray.max_steps) {
         ++ray.steps;
         float nx, ny, nz;
         ray.step(ray.lx, ray.ly, ray.lz, nx, ny, nz);
         if (RT ? outside_open(nx, ny, nz, nf) : outside(nx, ny, nz, nf)) break;
+        if (past_tail(ray, nx, ny, nz, tail)) break;
         int vx, vy, vz;
         segment_voxel1(ray.lx, ray.ly, ray.lz, nx, ny, nz, vx, vy, vz);
         if (occ.lookup(a.mask, a.blocks, a.bricks, n, vx, vy, vz)) {
@@ -113,6 +118,38 @@ hipError_t launch_brick_bits(const uint8_t* mask, int n, unsigned long long* bri
 __global__ void zero_words_kernel(uint32_t* __restrict__ p, int64_t n) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0u;
+}
+
+// Tail bits: one thread per 32-bit word of blocks; a block's bit is set when any block of its
+// 3x3x3 neighbourhood (clipped to the grid) is occupied.
+__global__ void tail_bits_kernel(const uint32_t* __restrict__ blocks, int n, uint32_t* __restrict__ tail) {
+    const int m = n / kTailCell, bpc = kTailCell / 8, nb = n >> 3;
+    const int64_t ncell = (int64_t)m * m * m;
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= (ncell + 31) >> 5) return;
+    uint32_t word = 0;
+    for (int k = 0; k < 32; ++k) {
+        const int64_t c = w * 32 + k;
+        if (c >= ncell) break;
+        const int cx = (int)(c % m), cy = (int)((c / m) % m), cz = (int)(c / ((int64_t)m * m));
+        bool any = false;
+        for (int bz = max(bpc * (cz - 1), 0); bz <= min(bpc * (cz + 2) - 1, nb - 1) && !any; ++bz)
+            for (int by = max(bpc * (cy - 1), 0); by <= min(bpc * (cy + 2) - 1, nb - 1) && !any; ++by)
+                for (int bx = max(bpc * (cx - 1), 0); bx <= min(bpc * (cx + 2) - 1, nb - 1); ++bx) {
+                    const int64_t b = ((int64_t)bz * nb + by) * nb + bx;
+                    any |= ((blocks[b >> 5] >> (b & 31)) & 1u) != 0;
+                }
+        if (any) word |= 1u << k;
+    }
+    tail[w] = word;
+}
+
+hipError_t launch_tail_bits(const uint32_t* blocks, int n, uint32_t* tail, hipStream_t s) {
+    const int64_t m = n / kTailCell;
+    const int64_t nwords = (m * m * m + 31) >> 5;
+    hipLaunchKernelGGL(tail_bits_kernel, dim3((unsigned)((nwords + 255) / 256)), dim3(256), 0, s,
+                       blocks, n, tail);
+    return hipGetLastError();
 }
 
 hipError_t launch_block_bits(const uint8_t* mask, int n, uint32_t* bits, hipStream_t s) {

@@ -87,7 +87,8 @@ constexpr int kAlphaRow = 8;       // p_in (3), p_out (3), next row of the lane 
 // of this batch (-1: none); each row's link names the lane's next row.
 template <bool AF>
 __device__ __forceinline__ void alpha_walk(RayWalk& ray, OccCache& occ, const RenderArgs& a,
-                                           float* queue, int lane, int& qn, int& head, int& last) {
+                                           float* queue, int lane, int& qn, int& head, int& last,
+                                           float tail) {
     const int n = a.n;
     const float nf = (float)n;
     while (qn < kAlphaFill) {
@@ -106,7 +107,7 @@ __device__ __forceinline__ void alpha_walk(RayWalk& ray, OccCache& occ, const Re
                                ray.sx, ray.sy, ray.sz, nx, ny, nz);
                 else
                     ray.step(ray.lx, ray.ly, ray.lz, nx, ny, nz);
-                if (outside(nx, ny, nz, nf)) {
+                if (outside(nx, ny, nz, nf) || past_tail(ray, nx, ny, nz, tail)) {
                     ray.alive = false;
                 } else {
                     int vx, vy, vz;
@@ -268,14 +269,15 @@ __global__ void __launch_bounds__(64, alpha_waves_per_simd<D>()) alpha_tile_kern
     Kahan S;
     float T = 1.0f;
     int qn = 0, head = -1, last = -1;
+    const float tail = tail_limit(a.tail, n, ray);
     const bool all_fast = !wave_any(ray.alive && !ray.fast);
     const int64_t n1 = n, n2 = (int64_t)n * n;
     const int hi = n - 1 - pad;
     while (true) {
         if (all_fast)
-            alpha_walk<true>(ray, occ, a, queue, lane, qn, head, last);
+            alpha_walk<true>(ray, occ, a, queue, lane, qn, head, last, tail);
         else
-            alpha_walk<false>(ray, occ, a, queue, lane, qn, head, last);
+            alpha_walk<false>(ray, occ, a, queue, lane, qn, head, last, tail);
         if (qn == 0) break;  // nothing queued and every lane done
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();

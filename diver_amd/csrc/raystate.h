@@ -423,6 +423,84 @@ __device__ __forceinline__ void gather_integral_at(const float* __restrict__ vox
     }
 }
 
+// ---- where a walk can end ----------------------------------------------------------------------
+// The tail bits are the 8^3-block occupancy dilated by one block: a block's bit is set when any
+// block of its 3x3x3 neighbourhood holds an occupied voxel.  A walk's positions stay within a
+// small fraction of a voxel of its ray's line (each step rounds once; one coordinate is snapped to
+// a plane), and the voxel it reads is within 1e-4 of a position, so a voxel the walk visits lies
+// within a few voxels of the line.  Once the line has left the last set block it meets, every
+// such voxel lies in an unset block's neighbourhood and is empty: nothing after that point is
+// composited, and the walk may end there with every output (intervals, colour, transmittance,
+// counts) unchanged.
+//
+// tail_limit runs a DDA over the (n/8)^3 blocks backwards, from the line's exit from the grid
+// towards the walk's start point, up to the first set block (the last one the walk meets), and
+// returns a bound on p.d past which the walk can end: kTailMargin voxels beyond that block's exit,
+// or -inf when no set block lies between the start and the exit (the walk can end at its first
+// step).  Without tail bits (NULL), for a dead ray, or should rounding carry the DDA out of the
+// grid before it reaches the start block, it returns +inf.
+constexpr int kTailCell = 8;         // voxels per block edge (the blocks of the block bits)
+constexpr float kTailMargin = 4.0f;  // voxels past the last set block's exit
+
+__device__ __forceinline__ float tail_limit(const uint32_t* __restrict__ tail, int n,
+                                            const RayWalk& ray) {
+    const float inf = __builtin_huge_valf();
+    if (!tail || !ray.alive) return inf;
+    constexpr float cs = (float)kTailCell, ics = 1.0f / kTailCell;
+    const int m = n / kTailCell;
+    const float nf = (float)n;
+    const float px = ray.lx, py = ray.ly, pz = ray.lz;
+    const float dx = ray.dx, dy = ray.dy, dz = ray.dz;
+    // the line's exit from the grid box, p0 + t_exit d
+    const float ax = dx > 0.0f ? (nf - px) / dx : dx < 0.0f ? -px / dx : inf;
+    const float ay = dy > 0.0f ? (nf - py) / dy : dy < 0.0f ? -py / dy : inf;
+    const float az = dz > 0.0f ? (nf - pz) / dz : dz < 0.0f ? -pz / dz : inf;
+    const float t_exit = fmaxf(fminf(ax, fminf(ay, az)), 0.0f);
+    const float qx = fmaf(dx, t_exit, px), qy = fmaf(dy, t_exit, py), qz = fmaf(dz, t_exit, pz);
+    // walk the blocks backwards from the exit (direction -d); s is the backward parameter
+    int cx = min(max((int)floorf(qx * ics), 0), m - 1);
+    int cy = min(max((int)floorf(qy * ics), 0), m - 1);
+    int cz = min(max((int)floorf(qz * ics), 0), m - 1);
+    float sx = dx < 0.0f ? ((cx + 1) * cs - qx) / -dx : dx > 0.0f ? (cx * cs - qx) / -dx : inf;
+    float sy = dy < 0.0f ? ((cy + 1) * cs - qy) / -dy : dy > 0.0f ? (cy * cs - qy) / -dy : inf;
+    float sz = dz < 0.0f ? ((cz + 1) * cs - qz) / -dz : dz > 0.0f ? (cz * cs - qz) / -dz : inf;
+    const float ex = dx != 0.0f ? cs / fabsf(dx) : inf;
+    const float ey = dy != 0.0f ? cs / fabsf(dy) : inf;
+    const float ez = dz != 0.0f ? cs / fabsf(dz) : inf;
+    const int stx = dx < 0.0f ? 1 : -1, sty = dy < 0.0f ? 1 : -1, stz = dz < 0.0f ? 1 : -1;
+    float s_in = 0.0f;
+    for (int it = 0; it < 3 * m + 3; ++it) {
+        const int c = (cz * m + cy) * m + cx;
+        if ((tail[c >> 5] >> (c & 31)) & 1u) {
+            // the last set block the line meets: its forward exit is at t_exit - s_in
+            const float dd = dx * dx + dy * dy + dz * dz;
+            return (px * dx + py * dy + pz * dz) + (t_exit - s_in) * dd + kTailMargin * sqrtf(dd);
+        }
+        const float s_out = fminf(sx, fminf(sy, sz));
+        if (s_out >= t_exit) return -inf;  // this block holds the start point: nothing set ahead
+        if (s_out == sx) {
+            cx += stx;
+            sx += ex;
+        } else if (s_out == sy) {
+            cy += sty;
+            sy += ey;
+        } else {
+            cz += stz;
+            sz += ez;
+        }
+        s_in = s_out;
+        if ((unsigned)cx >= (unsigned)m || (unsigned)cy >= (unsigned)m || (unsigned)cz >= (unsigned)m)
+            break;
+    }
+    return inf;  // left the grid before the start block (rounding): no bound
+}
+
+// The walk has passed its tail limit at position (nx, ny, nz).
+__device__ __forceinline__ bool past_tail(const RayWalk& ray, float nx, float ny, float nz,
+                                          float limit) {
+    return fmaf(nz, ray.dz, fmaf(ny, ray.dy, nx * ray.dx)) > limit;
+}
+
 constexpr int kSkip = 2;           // cheap steps per walk iteration over voxels known to be empty (0: off)
 
 // Cheap steps over voxels already known to be empty.  Before each full walk iteration a lane

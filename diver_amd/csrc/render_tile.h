@@ -283,6 +283,7 @@ struct TileLane {
     float reg = 0.0f;   // sparsity term: sum of log(1 + s^2 / 0.5) over the evaluated intervals
     int qn = 0;         // queued rows (wave-uniform)
     int iters = 0;      // phase-A iterations (wave-uniform)
+    float tail = 0.0f;  // the walk ends once p.d passes this (tail_limit)
 };
 
 // Phase A: every live lane walks until the queue holds kFill rows or every lane is done.  AF: every
@@ -314,7 +315,8 @@ __device__ __forceinline__ void phase_a(TileLane& L, const BatchArgs& p, const u
                                ray.sx, ray.sy, ray.sz, nx, ny, nz);
                 else
                     ray.step(ray.lx, ray.ly, ray.lz, nx, ny, nz);
-                if (RT ? outside_open(nx, ny, nz, nf) : outside(nx, ny, nz, nf)) {
+                if ((RT ? outside_open(nx, ny, nz, nf) : outside(nx, ny, nz, nf)) ||
+                    past_tail(ray, nx, ny, nz, L.tail)) {
                     ray.alive = false;
                 } else {
                     int vx, vy, vz;
@@ -341,7 +343,7 @@ __device__ __forceinline__ void phase_a(TileLane& L, const BatchArgs& p, const u
                         float sigma, alpha;
                         density_alpha<D>(s, sigma, alpha);
                         L.used += 1;
-                        if (p.sparsity) L.reg += log1pf(2.0f * s * s);
+                        if (p.sparsity) L.reg += sparsity_term(s);
                         if (alpha >= a.alpha_skip) {
                             wgt = L.T * alpha;
                             push = true;
@@ -368,8 +370,8 @@ __device__ __forceinline__ void phase_a(TileLane& L, const BatchArgs& p, const u
 // at most 128 VGPRs, which allows 4).  Hidden width 64 takes the 2-wave budget (the 32-channel
 // kernels compile to about 130 VGPRs and run 3 waves anyway; the FP32 and linear-density forms
 // need more than 168), except the 16-bit channel-0 kernels at 64 channels: held to 168 VGPRs they
-// park one value (the ray's output slot) in scratch, stored once before the walk and loaded once
-// after it, and shade a 256^3 x 64 scene 13 % faster than at 2 waves.
+// park the ray's output slot and request index (12 bytes) in scratch, stored once before the
+// walk and loaded once after it, and shade a 256^3 x 64 scene 13 % faster than at 2 waves.
 template <class D, int PREC, bool DCH0>
 constexpr int tile_waves_per_simd() {
     if (D::H <= 32) return 3;
@@ -415,6 +417,7 @@ __global__ void __launch_bounds__(64 * kWavesPerGroup, (tile_waves_per_simd<D, P
         L.ray.init_rt(ox, oy, oz, dx, dy, dz, a.xyzmin, a.voxel_size, n, a.octrees, valid);
     else
         L.ray.init(ox, oy, oz, dx, dy, dz, a.xyzmin, a.voxel_size, (float)n, valid);
+    L.tail = tail_limit(a.tail, n, L.ray);
 
     // Most tiles of a view miss the occupied cells entirely: such a wave skips the bitmap copy and
     // the view term (nothing would read them) and goes straight to writing T = 1.

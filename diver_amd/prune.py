@@ -30,6 +30,7 @@ class _WalkGrid:
         self.octrees = None
         self.blocks = None
         self.bricks = None
+        self.tail = None
 
 
 def alpha_map(grid: SceneGrid, decoder, o: torch.Tensor, d: torch.Tensor, pad: int = 0,

@@ -32,6 +32,8 @@ class SceneGrid:
                         if n % 64 == 0 else None)
         self.blocks = calls.block_bits(self.mask) if n % 8 == 0 else None
         self.bricks = calls.brick_bits(self.mask) if n % 8 == 0 else None
+        # where each walk may end (the block bits dilated by one block); no effect on results
+        self.tail = calls.tail_bits(self.blocks, n) if n % 8 == 0 else None
 
     @classmethod
     def from_synthetic(cls, scene, device) -> 'SceneGrid':

@@ -193,6 +193,9 @@ typedef struct diver_grid {
     int channels;
     float xyzmin;
     float voxel_size;
+    const uint32_t* tail;   /* tail bits from diver_tail_bits (of the same mask), or NULL: each
+                               walk ends once no occupied voxel can lie ahead on its ray, instead
+                               of at the grid's exit (no effect on results) */
 } diver_grid;
 
 enum diver_precision { DIVER_FP32 = 0, DIVER_BF16 = 1, DIVER_FP16 = 2 };
@@ -337,6 +340,15 @@ int diver_positional_encoding(int64_t n, const float* d, float* pe, diver_stream
  * ceil((n/8)^3 / 32) words.  No counterpart in the reference (an acceleration structure for
  * diver_render_*). */
 int diver_block_bits(const uint8_t* mask, int n, uint32_t* bits, diver_stream_t stream);
+
+/* Tail bits of a mask from its 8^3-block bits (diver_block_bits; n a multiple of 8): the block
+ * bits dilated by one block -- bit b of word b/32 is set when a block of block b's 3x3x3
+ * neighbourhood holds an occupied voxel (same block numbering); tail has ceil((n/8)^3 / 32)
+ * words.  A walk from p along d can end once its line has left the last set block it meets:
+ * every voxel it could still visit lies within a few voxels of the line, inside an unset block's
+ * neighbourhood, so empty.  No counterpart in the reference (an acceleration structure for
+ * diver_render_*, diver_alpha_map and diver_walk_rays). */
+int diver_tail_bits(const uint32_t* blocks, int n, uint32_t* tail, diver_stream_t stream);
 
 /* Bit-packed mask per 4^3 brick (n a multiple of 4): word b = (bz*(n/4) + by)*(n/4) + bx holds
  * bit (z&3)*16 + (y&3)*4 + (x&3) for voxel (x, y, z) of the brick; bricks has (n/4)^3 words.

@@ -206,3 +206,44 @@ def test_ray_suffix_matches_segmented_sums(dev):
     ray = torch.repeat_interleave(torch.arange(R), cnt.long())
     got_bg = calls.ray_suffix(off.to(dev), cnt.to(dev), w.to(dev), c.to(dev), bg.to(dev)).cpu().double()
     assert float((got_bg - (ref + bg.double()[ray][:, None])).abs().max()) <= 1e-5
+
+
+class _ByteMaskGrid:
+    """The same grid without its block/brick bits: diver_render_rows then takes the byte-mask
+    path, which walks, integrates every channel and composites per lane in one kernel."""
+
+    def __init__(self, grid):
+        self.voxels, self.mask, self.octrees = grid.voxels, grid.mask, grid.octrees
+        self.n, self.channels = grid.n, grid.channels
+        self.xyzmin, self.voxel_size = grid.xyzmin, grid.voxel_size
+        self.blocks = self.bricks = self.tail = None
+
+
+@pytest.mark.parametrize('traversal,t_stop', [('offline', 0.0), ('offline', 1e-4),
+                                              ('realtime', 1e-4)])
+def test_render_rows_match_byte_mask_path(dev, traversal, t_stop):
+    """The rows of the block/brick pipeline (walk writing segments, features gathered over all
+    rows, then a per-ray pass for s, T_before and alpha) equal the byte-mask kernel's, which
+    computes all of them on the walk: the same segments, integrals and transmittance, bit for
+    bit; and each ray's last row leaves the transmittance the forward returned."""
+    from diver_amd import calls
+    sc, _gt, o, d, layers = _setup(dev, n_rays=2048, seed=5)
+    grid = SceneGrid.from_synthetic(sc, dev)
+    model = FusedRenderer(grid, BenchmarkDecoder(layers, 32).to(dev), traversal=traversal)
+    og, dg = torch.from_numpy(o).to(dev), torch.from_numpy(d).to(dev)
+    opts = calls.render_opts('fp32', traversal, t_stop, 0.0)
+    rgba, count = calls.render_rays(grid, model.handle, og, dg, opts)
+    cnt64 = count.to(torch.int64)
+    off = torch.cumsum(cnt64, 0) - cnt64
+    K = int(cnt64.sum())
+    assert K > 1000
+    rows = calls.render_rows(grid, model.handle, og, dg, opts, count, off, K)
+    ref = calls.render_rows(_ByteMaskGrid(grid), model.handle, og, dg, opts, count, off, K)
+    assert torch.equal(rows, ref)
+    # T after each ray's last row, from its header (T_before, and sigma = softplus(s)) in the
+    # forward's FP32 order, equals the forward's T wherever the ray has rows
+    last = (off + cnt64 - 1)[cnt64 > 0]
+    s, tb = rows[last, 6].double(), rows[last, 7].double()
+    t_after = tb * torch.exp(-torch.nn.functional.softplus(s))
+    t_fwd = rgba[cnt64 > 0, 3].double()
+    assert float((t_after - t_fwd).abs().max()) <= 1e-5

@@ -46,15 +46,16 @@ def test_tile_kernels_instantiated(report):
 def test_matrix_core_tile_kernels_do_not_spill(report):
     # ...EEELi<PREC>ELb<RT>ELb<DCH0>EEEv: PREC 1 = BF16, 2 = FP16; DCH0 = density from channel 0
     # (every decoder the models ship).  The generic linear-density form is held to the scratch bound.
-    # The 64-channel kernels run 3 waves per SIMD with the ray's output slot (at most 8 bytes) in
-    # scratch, written once before the walk and read once after it; nothing else may spill.
+    # The 64-channel kernels run 3 waves per SIMD with the ray's output slot and request (at most
+    # 12 bytes) in scratch, written once before the walk and read once after it; nothing else may
+    # spill.
     checked = 0
     for name, r in _tile_kernels(report).items():
         tail = name.split('EEELi')[1]
         prec, dch0 = tail[0], tail.split('ELb')[2][0]
         if prec in '12' and dch0 == '1':
             if 'DecILi64ELi64' in name:
-                assert r['vgpr'] <= 168 and r['vgpr_spill'] <= 2 and r['scratch_bytes'] <= 16, (name, r)
+                assert r['vgpr'] <= 168 and r['vgpr_spill'] <= 3 and r['scratch_bytes'] <= 16, (name, r)
             else:
                 assert r['vgpr_spill'] == 0 and r['scratch_bytes'] == 0, (name, r)
             checked += 1
