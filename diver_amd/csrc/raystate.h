@@ -112,8 +112,9 @@ struct RayWalk {
             if (outside(px, py, pz, (float)m)) return;
             int vx, vy, vz;
             segment_voxel1(qx, qy, qz, px, py, pz, vx, vy, vz);
-            const int m3 = m * m * m;
-            const int ci = (vz * m + vy) * m + vx;
+            // |vx|, |vy|, |vz| <= m + 1 <= 1025: the 24-bit multiplies are exact (full rate)
+            const int m3 = __mul24(__mul24(m, m), m);
+            const int ci = __mul24(__mul24(vz, m) + vy, m) + vx;
             const int gi = off + ci;
             const bool occ = (ci >= 0 && ci < m3 && gi < total) ? octrees[gi] != 0 : false;
             if (occ) {
@@ -470,7 +471,7 @@ __device__ __forceinline__ float tail_limit(const uint32_t* __restrict__ tail, i
     const int stx = dx < 0.0f ? 1 : -1, sty = dy < 0.0f ? 1 : -1, stz = dz < 0.0f ? 1 : -1;
     float s_in = 0.0f;
     for (int it = 0; it < 3 * m + 3; ++it) {
-        const int c = (cz * m + cy) * m + cx;
+        const int c = __mul24(__mul24(cz, m) + cy, m) + cx;
         if ((tail[c >> 5] >> (c & 31)) & 1u) {
             // the last set block the line meets: its forward exit is at t_exit - s_in
             const float dd = dx * dx + dy * dy + dz * dz;
