@@ -18,7 +18,8 @@ EXPORTS = [
     'diver_aabb_intersect', 'diver_ray_march', 'diver_integrate', 'diver_integrate_backward',
     'diver_composite', 'diver_composite_backward', 'diver_decoder_create', 'diver_decoder_destroy',
     'diver_decoder_param_count', 'diver_mlp_eval', 'diver_render_rays', 'diver_render_camera',
-    'diver_camera_rays', 'diver_block_bits', 'diver_tail_bits', 'diver_walk_rays', 'diver_render_rows',
+    'diver_camera_rays', 'diver_camera_rays_pinhole', 'diver_block_bits', 'diver_tail_bits',
+    'diver_walk_rays', 'diver_render_rows',
     'diver_decoder_update', 'diver_brick_bits', 'diver_render_batch', 'diver_alpha_map',
     'diver_corner_keys', 'diver_integrate_table', 'diver_integrate_table_backward',
     'diver_adam_step', 'diver_decoder_update_range', 'diver_ray_suffix',
@@ -81,7 +82,8 @@ class FrameInfo(ctypes.Structure):
 class Request(ctypes.Structure):
     _fields_ = [('c2w', ctypes.c_float * 12), ('width', ctypes.c_int), ('height', ctypes.c_int),
                 ('focal', ctypes.c_float), ('n_rays', ctypes.c_int64), ('o', ctypes.c_void_p),
-                ('d', ctypes.c_void_p), ('rgba', ctypes.c_void_p), ('count', ctypes.c_void_p)]
+                ('d', ctypes.c_void_p), ('rgba', ctypes.c_void_p), ('count', ctypes.c_void_p),
+                ('fy', ctypes.c_float), ('cx', ctypes.c_float), ('cy', ctypes.c_float)]
 
 
 def _declare(L):
@@ -113,6 +115,8 @@ def _declare(L):
         'diver_render_camera': [ctypes.POINTER(Grid), vp, ctypes.POINTER(RenderOpts),
                                 ctypes.POINTER(ctypes.c_float), i32, i32, fl, vp, vp, vp],
         'diver_camera_rays': [ctypes.POINTER(ctypes.c_float), i32, i32, fl, vp, vp, vp],
+        'diver_camera_rays_pinhole': [ctypes.POINTER(ctypes.c_float), i32, i32, fl, fl, fl, fl, vp, vp,
+                                      vp],
         'diver_block_bits': [vp, i32, vp, vp],
         'diver_tail_bits': [vp, i32, vp, vp],
         'diver_brick_bits': [vp, i32, vp, vp],

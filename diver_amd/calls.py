@@ -461,7 +461,8 @@ def render_camera(grid, decoder, c2w, width, height, focal, opts, with_count=Tru
 def render_batch(grid, decoder, requests, opts, with_count=True):
     """Render several requests in one launch.
 
-    requests: list of dicts, either {'c2w', 'width', 'height', 'focal'} (camera) or
+    requests: list of dicts, either {'c2w', 'width', 'height', 'focal'} (camera; with 'fy',
+    'cx', 'cy' as well a general pinhole camera with fx = focal, see camera_rays_pinhole) or
     {'o', 'd'} (explicit rays [R,3]).  Returns a list of (rgba, count) in request order, rgba
     [H,W,4] / [R,4] as render_camera / render_rays return them.
     """
@@ -479,6 +480,8 @@ def render_batch(grid, decoder, requests, opts, with_count=True):
             for k, x in enumerate(list(rq['c2w'].reshape(-1))[:12]):
                 r.c2w[k] = float(x)
             r.width, r.height, r.focal = W, H, float(rq['focal'])
+            if 'fy' in rq:
+                r.fy, r.cx, r.cy = float(rq['fy']), float(rq['cx']), float(rq['cy'])
         else:
             _dev(grid.voxels, rq['o'], rq['d'])
             o, d = _f32(rq['o'], 'o'), _f32(rq['d'], 'd')
@@ -504,4 +507,16 @@ def camera_rays(c2w, width, height, focal, device):
     m = (ctypes.c_float * 12)(*[float(x) for x in list(c2w.reshape(-1))])
     _lib.call('diver_camera_rays', m, int(width), int(height), float(focal), o.data_ptr(),
               d.data_ptr(), torch.cuda.current_stream(o.device).cuda_stream)
+    return o, d
+
+
+def camera_rays_pinhole(c2w, width, height, fx, fy, cx, cy, device):
+    """Rays of a general pinhole camera (utils/dataset/tanks.py:120-169): pixel (x, y) looks along
+    ((x + 0.5 - cx)/fx, -(y + 0.5 - cy)/fy, -1) rotated by c2w[:, :3], from c2w[:, 3]."""
+    o = torch.empty((height * width, 3), dtype=torch.float32, device=device)
+    d = torch.empty((height * width, 3), dtype=torch.float32, device=device)
+    m = (ctypes.c_float * 12)(*[float(x) for x in list(c2w.reshape(-1))])
+    _lib.call('diver_camera_rays_pinhole', m, int(width), int(height), float(fx), float(fy),
+              float(cx), float(cy), o.data_ptr(), d.data_ptr(),
+              torch.cuda.current_stream(o.device).cuda_stream)
     return o, d

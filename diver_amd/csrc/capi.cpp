@@ -698,6 +698,17 @@ int diver_alpha_map(const diver_grid* grid, const diver_decoder* dec, int64_t n_
                       "alpha_map");
 }
 
+// The Blender camera (utils/dataset/ray_utils.py): fx = fy = focal, principal point at the centre.
+static void blender_camera(diver::CameraDesc& cam, const float* c2w, int width, int height,
+                           float focal) {
+    std::memcpy(cam.c2w, c2w, sizeof(cam.c2w));
+    cam.width = width;
+    cam.height = height;
+    cam.fx = cam.fy = focal;
+    cam.cx = 0.5f * (float)width;
+    cam.cy = 0.5f * (float)height;
+}
+
 int diver_render_camera(const diver_grid* grid, const diver_decoder* dec,
                         const diver_render_opts* opts, const float* c2w_host, int width,
                         int height, float focal, float* rgba, int32_t* count,
@@ -707,10 +718,7 @@ int diver_render_camera(const diver_grid* grid, const diver_decoder* dec,
     if (st != DIVER_OK) return st;
     REQUIRE(c2w_host && rgba, "null pointer");
     REQUIRE(width > 0 && height > 0 && focal > 0, "bad camera");
-    std::memcpy(a.cam.c2w, c2w_host, sizeof(a.cam.c2w));
-    a.cam.width = width;
-    a.cam.height = height;
-    a.cam.focal = focal;
+    blender_camera(a.cam, c2w_host, width, height, focal);
     int64_t tiles = (int64_t)((width + 7) / 8) * ((height + 7) / 8);
     a.n_rays = tiles * 64;
     a.rgba = rgba;
@@ -739,11 +747,13 @@ int diver_render_batch(const diver_grid* grid, const diver_decoder* dec,
         diver::ReqDesc& q = p.req[i];
         REQUIRE(r.rgba || (r.width == 0 && r.n_rays == 0), "request %d: null rgba", i);
         if (r.width > 0) {
-            REQUIRE(r.height > 0 && r.focal > 0, "request %d: bad camera", i);
-            std::memcpy(q.cam.c2w, r.c2w, sizeof(q.cam.c2w));
-            q.cam.width = r.width;
-            q.cam.height = r.height;
-            q.cam.focal = r.focal;
+            REQUIRE(r.height > 0 && r.focal > 0 && r.fy >= 0, "request %d: bad camera", i);
+            blender_camera(q.cam, r.c2w, r.width, r.height, r.focal);
+            if (r.fy > 0) {  // a general pinhole camera (Tanks and Temples, BlendedMVS)
+                q.cam.fy = r.fy;
+                q.cam.cx = r.cx;
+                q.cam.cy = r.cy;
+            }
             q.n_rays = (int64_t)((r.width + 7) / 8) * ((r.height + 7) / 8) * 64;
         } else {
             REQUIRE(r.width == 0, "request %d: negative width", i);
@@ -876,11 +886,20 @@ int diver_camera_rays(const float* c2w_host, int width, int height, float focal,
     REQUIRE(c2w_host && o && d, "null pointer");
     REQUIRE(width > 0 && height > 0 && focal > 0, "bad camera");
     diver::CameraDesc cam{};
-    std::memcpy(cam.c2w, c2w_host, sizeof(cam.c2w));
-    cam.width = width;
-    cam.height = height;
-    cam.focal = focal;
+    blender_camera(cam, c2w_host, width, height, focal);
     return hip_status(diver::launch_camera_rays(cam, o, d, S(stream)), "camera_rays");
+}
+
+int diver_camera_rays_pinhole(const float* c2w_host, int width, int height, float fx, float fy,
+                              float cx, float cy, float* o, float* d, diver_stream_t stream) {
+    REQUIRE(c2w_host && o && d, "null pointer");
+    REQUIRE(width > 0 && height > 0 && fx > 0 && fy > 0, "bad camera");
+    diver::CameraDesc cam{};
+    blender_camera(cam, c2w_host, width, height, fx);
+    cam.fy = fy;
+    cam.cx = cx;
+    cam.cy = cy;
+    return hip_status(diver::launch_camera_rays(cam, o, d, S(stream)), "camera_rays_pinhole");
 }
 
 }  // extern "C"

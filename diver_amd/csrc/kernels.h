@@ -9,7 +9,8 @@ struct CameraDesc {
     float c2w[12];  // row-major 3x4 camera-to-world
     int width;      // 0 = explicit rays
     int height;
-    float focal;
+    float fx, fy;   // focal lengths in pixels
+    float cx, cy;   // principal point in pixels (width / 2, height / 2 for the Blender camera)
 };
 
 struct RenderArgs {

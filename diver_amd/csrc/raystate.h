@@ -342,12 +342,16 @@ __device__ __forceinline__ void gather_integral(const float* __restrict__ voxels
     }
 }
 
-// Pinhole camera ray of pixel (x, y) (FP32): utils/dataset/ray_utils.py:5-56 conventions.
+// Pinhole camera ray of pixel (x, y) (FP32): pixel centres at +0.5, camera space
+// ((x + 0.5 - cx) / fx, -(y + 0.5 - cy) / fy, -1), rotated by c2w and normalised, from the camera
+// centre.  With fx = fy = focal and (cx, cy) = (W/2, H/2) this is utils/dataset/ray_utils.py:5-56
+// (the Blender camera); otherwise the Tanks and Temples / BlendedMVS camera of
+// utils/dataset/tanks.py:120-169 (whose poses and origins are converted on the host).
 __device__ __forceinline__ void camera_ray(const CameraDesc& cam, int x, int y, float& ox, float& oy,
                                            float& oz, float& dx, float& dy, float& dz) {
 #pragma clang fp contract(off)
-    float cx = ((float)x + 0.5f - 0.5f * (float)cam.width) / cam.focal;
-    float cy = -(((float)y + 0.5f - 0.5f * (float)cam.height) / cam.focal);
+    float cx = ((float)x + 0.5f - cam.cx) / cam.fx;
+    float cy = -(((float)y + 0.5f - cam.cy) / cam.fy);
     float cz = -1.0f;
     const float* m = cam.c2w;  // row-major 3x4
     float wx = m[0] * cx + m[1] * cy + m[2] * cz;

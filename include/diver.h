@@ -253,7 +253,10 @@ int diver_render_camera(const diver_grid* grid, const diver_decoder* dec,
 
 /* One request of diver_render_batch: a pinhole camera (width > 0; c2w row-major 3x4, copied at
  * the call) or n_rays explicit rays o, d [n_rays][3] (width == 0).  Outputs as
- * diver_render_camera / diver_render_rays. */
+ * diver_render_camera / diver_render_rays.  The camera is the Blender one of diver_render_camera
+ * (focal, principal point at the image centre) while fy is 0 (a zero-initialised request); with
+ * fy > 0 it is the general pinhole camera of diver_camera_rays_pinhole with fx = focal and
+ * principal point (cx, cy): the Tanks and Temples / BlendedMVS camera. */
 typedef struct diver_request {
     float c2w[12];
     int width;
@@ -264,6 +267,9 @@ typedef struct diver_request {
     const float* d;
     float* rgba;
     int32_t* count;
+    float fy;
+    float cx;
+    float cy;
 } diver_request;
 
 #define DIVER_MAX_REQUESTS 16
@@ -395,6 +401,15 @@ int diver_walk_rays(const diver_grid* grid, int traversal, int64_t n_rays, const
 /* The FP32 rays diver_render_camera traces: o, d [height*width][3]. */
 int diver_camera_rays(const float* c2w_host, int width, int height, float focal, float* o,
                       float* d, diver_stream_t stream);
+
+/* The rays of a general pinhole camera: pixel (x, y) looks along
+ * ((x + 0.5 - cx)/fx, -(y + 0.5 - cy)/fy, -1) in camera space, rotated by c2w and normalised,
+ * from c2w's translation (utils/dataset/tanks.py:120-169, get_ray_directions / get_rays).  The
+ * dataset's pose conversion (parse_extrinsics: OpenCV axes to these, tanks.py:172-181), its
+ * intrinsics scaling (tanks.py:27-42) and the origin normalisation (o - mid) / scale
+ * (tanks.py:46-52, 98) are applied to c2w on the host (diver_amd.cameras). */
+int diver_camera_rays_pinhole(const float* c2w_host, int width, int height, float fx, float fy,
+                              float cx, float cy, float* o, float* d, diver_stream_t stream);
 
 /* ---- render server ---------------------------------------------------------------------------- */
 

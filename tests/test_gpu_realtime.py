@@ -1,7 +1,7 @@
 """The packaged real-time frame loop (diver_amd.realtime.RealTimeRenderer: eval_rt.py:106-122 on
 the drop-in aabb_intersect / ray_march / mlp_eval ops, the plain FP32 GPU path) and on-device
-camera rays (diver_camera_rays, utils/dataset/ray_utils.py:5-56 conventions) against references
-computed on the CPU.
+camera rays (diver_camera_rays, utils/dataset/ray_utils.py:5-56 conventions; the general pinhole
+diver_camera_rays_pinhole, utils/dataset/tanks.py:120-169) against references computed on the CPU.
 
 Stated tolerances:
   camera rays: origins equal the camera centre exactly; directions within 4e-7 (a few ulp of a
@@ -67,6 +67,65 @@ def test_camera_rays_closed_forms(dev):
     v = np.array([-(W - 1) / 2, (H - 1) / 2, -focal])
     np.testing.assert_allclose(cam[0, 0], v / np.linalg.norm(v), atol=3e-7)
     assert abs(math.atan((W / 2) / focal) - fov / 2) < 1e-12
+
+
+def _tanks_fp64(c2w, W, H, fx, fy, cx, cy):
+    """utils/dataset/tanks.py:120-169 (get_ray_directions with (fx, fy, cx, cy), get_rays) in
+    float64."""
+    i = (np.arange(W, dtype=np.float64) + 0.5)[None, :].repeat(H, 0)
+    j = (np.arange(H, dtype=np.float64) + 0.5)[:, None].repeat(W, 1)
+    dirs = np.stack([(i - cx) / fx, -(j - cy) / fy, -np.ones_like(i)], -1)
+    d = dirs @ np.asarray(c2w, np.float64)[:, :3].T
+    d /= np.linalg.norm(d, axis=-1, keepdims=True)
+    return d.reshape(-1, 3)
+
+
+def test_pinhole_camera_rays_follow_tanks(cameras, dev):
+    """A Tanks and Temples view (diver_amd.cameras.tanks_camera: OpenCV pose, off-centre principal
+    point, fx != fy, origin normalised into the scene box) against tanks.py's formula in FP64:
+    directions within 4e-7, origins equal to the normalised camera centre; with fx = fy = focal and
+    the centre as principal point the pinhole rays are the Blender rays bit for bit."""
+    from diver_amd import cameras as CM
+    pose = np.eye(4, dtype=np.float32)
+    pose[:3, :3] = np.asarray(S.look_at_c2w(np.array([0.3, -2.5, 1.0]))[:, :3], np.float32)
+    pose[:3, 1:3] *= -1  # an OpenCV pose of the same camera
+    pose[:3, 3] = [1.3, -7.5, 3.0]
+    intr = CM.scale_intrinsics(1163.7, 1158.2, 971.4, 531.9, 'TanksAndTemple', (1920, 1080))
+    req = CM.tanks_camera(pose, intr, (-3, -3, -3, 3, 3, 3), (1920, 1080))
+    o, d = calls.camera_rays_pinhole(req['c2w'], 1920, 1080, req['focal'], req['fy'], req['cx'],
+                                     req['cy'], dev)
+    o, d = o.cpu().numpy(), d.cpu().numpy()
+    assert np.array_equal(o, np.broadcast_to(req['c2w'][:, 3], o.shape))
+    want = _tanks_fp64(req['c2w'], 1920, 1080, req['focal'], req['fy'], req['cx'], req['cy'])
+    assert np.abs(d - want).max() < 4e-7
+    c2w = np.asarray(cameras[57], np.float32)
+    f = S.focal_for(800, 0.69)
+    ob, db = calls.camera_rays(c2w, 800, 600, f, dev)
+    op, dp = calls.camera_rays_pinhole(c2w, 800, 600, f, f, 400.0, 300.0, dev)
+    assert torch.equal(ob, op) and torch.equal(db, dp)
+
+
+def test_pinhole_request_renders_its_rays(scene64, dev):
+    """A pinhole request in diver_render_batch renders each pixel as diver_render_rays renders
+    that pixel's pinhole ray (bit for bit), and a zero fy keeps the Blender camera."""
+    from diver_amd.renderer import SceneGrid
+    grid = SceneGrid.from_synthetic(scene64, dev)
+    dec = Decoder(from_baseline(S.baseline_decoder_layers(32, 32), 32))
+    c2w = S.look_at_c2w(np.array([0.4, -2.2, 1.1])).astype(np.float32)
+    W, H, fx, fy, cx, cy = 96, 72, 80.0, 86.0, 41.3, 39.8
+    opts = calls.render_opts('bf16', 'realtime', 1e-4, 0.0)
+    (rgba, cnt), = calls.render_batch(grid, dec, [{'c2w': c2w, 'width': W, 'height': H,
+                                                    'focal': fx, 'fy': fy, 'cx': cx, 'cy': cy}],
+                                      opts)
+    o, d = calls.camera_rays_pinhole(c2w, W, H, fx, fy, cx, cy, dev)
+    ref, rcnt = calls.render_rays(grid, dec, o, d, opts)
+    assert torch.equal(rgba.reshape(-1, 4), ref) and torch.equal(cnt.reshape(-1), rcnt)
+    assert int(rcnt.sum()) > 100
+    (rb, _), = calls.render_batch(grid, dec, [{'c2w': c2w, 'width': W, 'height': H,
+                                               'focal': fx, 'fy': 0.0, 'cx': 0.0, 'cy': 0.0}], opts)
+    ob, db = calls.camera_rays(c2w, W, H, fx, dev)
+    refb, _ = calls.render_rays(grid, dec, ob, db, opts)
+    assert torch.equal(rb.reshape(-1, 4), refb)
 
 
 def _check_frame(rgb, ref, near_limit=0.01, tol=2e-4):
