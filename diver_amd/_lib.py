@@ -24,7 +24,8 @@ EXPORTS = [
     'diver_corner_keys', 'diver_integrate_table', 'diver_integrate_table_backward',
     'diver_adam_step', 'diver_decoder_update_range', 'diver_ray_suffix',
     'diver_render_rays_sparsity', 'diver_server_clock', 'diver_server_create',
-    'diver_server_destroy', 'diver_server_add_scene', 'diver_server_submit', 'diver_server_wait',
+    'diver_server_destroy', 'diver_server_add_scene', 'diver_server_submit',
+    'diver_server_submit_pinhole', 'diver_server_wait',
     'diver_server_take_completed', 'diver_server_history', 'diver_adam_step_rows',
     'diver_adjoint_decoder_forward', 'diver_adjoint_decoder_backward', 'diver_positional_encoding',
 ]
@@ -145,6 +146,8 @@ def _declare(L):
                                    vp, ctypes.POINTER(ctypes.c_int)],
         'diver_server_submit': [vp, i32, ctypes.POINTER(ctypes.c_float), i32, i32, fl, vp, vp, vp,
                                 ctypes.POINTER(ctypes.c_int64)],
+        'diver_server_submit_pinhole': [vp, i32, ctypes.POINTER(ctypes.c_float), i32, i32, fl, fl, fl,
+                                        fl, vp, vp, vp, ctypes.POINTER(ctypes.c_int64)],
         'diver_server_wait': [vp, i64, ctypes.c_double, ctypes.POINTER(FrameInfo)],
     })
     for name, args in sig.items():

@@ -61,6 +61,7 @@ struct Frame {
     diver_frame_info info;
     float c2w[12];
     float focal;
+    float fy, cx, cy;  // a general pinhole camera when fy > 0 (diver_request's fields)
     float* rgba;
     int32_t* count;
     int64_t bytes;
@@ -213,6 +214,9 @@ void launch_loop(diver_server* s) {
             reqs[i].width = f->info.width;
             reqs[i].height = f->info.height;
             reqs[i].focal = f->focal;
+            reqs[i].fy = f->fy;
+            reqs[i].cx = f->cx;
+            reqs[i].cy = f->cy;
             reqs[i].rgba = f->rgba;
             reqs[i].count = f->count;
             if (he == hipSuccess) he = hipStreamWaitEvent(st, f->after, 0);
@@ -389,14 +393,17 @@ int diver_server_add_scene(diver_server* s, const diver_grid* grid, const diver_
     return DIVER_OK;
 }
 
-int diver_server_submit(diver_server* s, int scene_id, const float* c2w_host, int width,
-                        int height, float focal, float* rgba, int32_t* count,
-                        diver_stream_t after, int64_t* ticket) {
+static int submit_frame(diver_server* s, int scene_id, const float* c2w_host, int width,
+                        int height, float focal, float fy, float cx, float cy, float* rgba,
+                        int32_t* count, diver_stream_t after, int64_t* ticket) {
     if (!s || !c2w_host || !rgba || !ticket) return err(DIVER_EINVAL, "null argument");
-    if (width <= 0 || height <= 0 || !(focal > 0)) return err(DIVER_EINVAL, "bad camera");
+    if (width <= 0 || height <= 0 || !(focal > 0) || !(fy >= 0)) return err(DIVER_EINVAL, "bad camera");
     auto* f = new Frame();
     std::memcpy(f->c2w, c2w_host, sizeof(f->c2w));
     f->focal = focal;
+    f->fy = fy;
+    f->cx = cx;
+    f->cy = cy;
     f->rgba = rgba;
     f->count = count;
     f->bytes = (int64_t)width * height * kBytesPerPixel;
@@ -451,6 +458,21 @@ int diver_server_submit(diver_server* s, int scene_id, const float* c2w_host, in
     *ticket = f->info.ticket;
     s->cv_launch.notify_one();
     return DIVER_OK;
+}
+
+int diver_server_submit(diver_server* s, int scene_id, const float* c2w_host, int width,
+                        int height, float focal, float* rgba, int32_t* count,
+                        diver_stream_t after, int64_t* ticket) {
+    return submit_frame(s, scene_id, c2w_host, width, height, focal, 0.0f, 0.0f, 0.0f, rgba, count,
+                        after, ticket);
+}
+
+int diver_server_submit_pinhole(diver_server* s, int scene_id, const float* c2w_host, int width,
+                                int height, float fx, float fy, float cx, float cy, float* rgba,
+                                int32_t* count, diver_stream_t after, int64_t* ticket) {
+    if (!(fy > 0)) return err(DIVER_EINVAL, "bad camera");
+    return submit_frame(s, scene_id, c2w_host, width, height, fx, fy, cx, cy, rgba, count, after,
+                        ticket);
 }
 
 int diver_server_wait(diver_server* s, int64_t ticket, double timeout_s, diver_frame_info* info) {

@@ -192,9 +192,11 @@ class RenderServer:
 
     # ---- requests -------------------------------------------------------------------------
     def submit(self, scene: str, c2w, width: int, height: int, fov: float = 0.69,
-               out=None) -> FrameRequest:
+               out=None, intrinsics=None) -> FrameRequest:
         """Queue a frame.  ``out`` = (rgba [H, W, 4] float32, count [H, W] int32) device buffers to
-        render into (a client that reuses its buffers); by default new ones are allocated."""
+        render into (a client that reuses its buffers); by default new ones are allocated.
+        ``intrinsics`` = (fx, fy, cx, cy) selects a general pinhole camera (Tanks and Temples,
+        BlendedMVS: diver_amd.cameras) instead of the Blender camera of ``fov``."""
         sc = self.scenes.get(scene)
         if sc is None:
             raise KeyError(f'unknown scene {scene!r}')
@@ -212,12 +214,20 @@ class RenderServer:
                     raise ValueError('out must be contiguous rgba [H, W, 4] float32 and count '
                                      '[H, W] int32 tensors on the server\'s device')
             ticket = ctypes.c_int64(0)
+            m = c2w.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
             # the buffers' memory may have been used by work still queued on the current stream
-            st, msg = _lib.status_call(
-                'diver_server_submit', self._handle, sc.scene_id,
-                c2w.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), int(width), int(height),
-                float(focal_for(width, fov)), rgba.data_ptr(), count.data_ptr(),
-                torch.cuda.current_stream(self.device).cuda_stream, ctypes.byref(ticket))
+            stream = torch.cuda.current_stream(self.device).cuda_stream
+            if intrinsics is None:
+                st, msg = _lib.status_call(
+                    'diver_server_submit', self._handle, sc.scene_id, m, int(width), int(height),
+                    float(focal_for(width, fov)), rgba.data_ptr(), count.data_ptr(), stream,
+                    ctypes.byref(ticket))
+            else:
+                fx, fy, cx, cy = (float(v) for v in intrinsics)
+                st, msg = _lib.status_call(
+                    'diver_server_submit_pinhole', self._handle, sc.scene_id, m, int(width),
+                    int(height), fx, fy, cx, cy, rgba.data_ptr(), count.data_ptr(), stream,
+                    ctypes.byref(ticket))
         if st == _lib.DIVER_EBUDGET:
             raise AdmissionError(msg)
         if st != 0:

@@ -471,6 +471,11 @@ int diver_server_add_scene(diver_server* srv, const diver_grid* grid, const dive
 int diver_server_submit(diver_server* srv, int scene_id, const float* c2w_host, int width,
                         int height, float focal, float* rgba, int32_t* count,
                         diver_stream_t after, int64_t* ticket);
+/* The same for a general pinhole camera (fx, fy, principal point (cx, cy); see
+ * diver_camera_rays_pinhole): Tanks and Temples / BlendedMVS views. */
+int diver_server_submit_pinhole(diver_server* srv, int scene_id, const float* c2w_host, int width,
+                                int height, float fx, float fy, float cx, float cy, float* rgba,
+                                int32_t* count, diver_stream_t after, int64_t* ticket);
 /* Blocks until the frame is complete (or timeout_s seconds pass: DIVER_ETIMEDOUT; timeout_s < 0
  * waits without limit) and fills *info (may be NULL).  A completed frame is waited for at most
  * once. */

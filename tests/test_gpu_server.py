@@ -66,6 +66,35 @@ def test_server_mixed_frames_equal_alone(cameras, dev):
         assert torch.equal(r.count, count), (sc, w, h)
 
 
+def test_server_pinhole_frames_equal_alone(cameras, dev):
+    """Tanks and Temples style frames (principal point off the centre, fx != fy) through the
+    server equal the same pinhole requests rendered alone, and Blender frames in the same launches
+    keep their camera."""
+    (ga, da), _ = _scenes(dev)
+    intr = [(70.0, 76.0, 30.2, 27.7), (55.5, 50.0, 12.0, 31.4)]
+    with RenderServer(dev, n_streams=2) as srv:
+        srv.add_scene('a', ga, da, precision='bf16')
+        reqs = []
+        for i in range(6):
+            cam = cameras[(11 * i) % len(cameras)]
+            if i % 2:
+                reqs.append(srv.submit('a', cam, 64, 48))
+            else:
+                reqs.append(srv.submit('a', cam, 64, 48, intrinsics=intr[(i // 2) % 2]))
+        for r in reqs:
+            r.result(timeout=60)
+    torch.cuda.synchronize()
+    opts = calls.render_opts('bf16', 'realtime', 1e-4, 0.0)
+    for i, r in enumerate(reqs):
+        c2w = np.asarray(cameras[(11 * i) % len(cameras)], np.float32)
+        rq = {'c2w': c2w, 'width': 64, 'height': 48, 'focal': focal_for(64, 0.69)}
+        if i % 2 == 0:
+            fx, fy, cx, cy = intr[(i // 2) % 2]
+            rq.update(focal=fx, fy=fy, cx=cx, cy=cy)
+        (rgba, count), = calls.render_batch(ga, da, [rq], opts)
+        assert torch.equal(r.rgba, rgba) and torch.equal(r.count, count), i
+
+
 def test_server_admission(cameras, dev):
     (ga, da), _ = _scenes(dev)
     need = ga.nbytes + ga.blocks.numel() * 4 + ga.bricks.numel() * 8 + da.nbytes
