@@ -65,7 +65,8 @@ def _edge_rays(n_rays, seed):
 
 
 def test_tail_bits_match_definition(dev):
-    for sc in (S.config1_scene(), S.config2_scene(32, 256)):
+    for sc in (S.config1_scene(), S.config2_scene(32, 256), S.make_scene(192, 8, 5, 2, seed=3),
+               S.make_scene(40, 8, 3, 1, seed=4)):
         grid = SceneGrid.from_synthetic(sc, dev)
         got = grid.tail.cpu().numpy().view(np.uint32)
         assert np.array_equal(got, _tail_numpy(sc.mask.reshape((grid.n,) * 3).astype(bool)))
@@ -96,6 +97,27 @@ def test_random_rays_unchanged(dev, scene256, traversal, t_stop):
     for x, y in zip(a, b):
         assert torch.equal(x, y)
     assert int(a[1].sum()) > 10000
+
+
+@pytest.mark.parametrize('n', [192, 40])
+def test_other_sizes_unchanged(dev, n):
+    """Grids whose size is no power of two (the real-time walk needs n % 64 == 0, so 40 runs the
+    offline walk only): the same intervals and renders with and without the tail bits."""
+    sc = S.make_scene(n, 32, 5, 2, seed=3)
+    grid = SceneGrid.from_synthetic(sc, dev)
+    dec = Decoder(from_baseline(S.baseline_decoder_layers(32, 32), 32))
+    o, d = _edge_rays(8192, 13)
+    og, dg = torch.from_numpy(o).to(dev), torch.from_numpy(d).to(dev)
+    walks = ['offline', 'realtime'] if n % 64 == 0 else ['offline']
+    for traversal in walks:
+        ia, na = calls.walk_rays(grid, og, dg, traversal, max_n=3 * n)
+        ib, nb = calls.walk_rays(_without_tail(grid), og, dg, traversal, max_n=3 * n)
+        assert torch.equal(na, nb) and torch.equal(ia, ib), traversal
+        assert int(na.sum()) > 1000
+        opts = calls.render_opts('fp32', traversal, 1e-4, 0.0)
+        a = calls.render_rays(grid, dec, og, dg, opts)
+        b = calls.render_rays(_without_tail(grid), dec, og, dg, opts)
+        assert torch.equal(a[0], b[0]) and torch.equal(a[1], b[1]), traversal
 
 
 @pytest.mark.parametrize('traversal', ['offline', 'realtime'])
