@@ -100,10 +100,10 @@ def test_batch_limits(scene64, cameras, dev):
 
 @pytest.mark.parametrize('precision', ['bf16', 'fp32'])
 def test_rays_per_wave_does_not_change_results(scene64, cameras, dev, precision):
-    """Small launches give each wave fewer rays (64 at >= 131,072 rays, then 32, 16, 8 down to
-    about 2,048 waves).  The same 4,096 rays rendered in launches of 4,096, 40,000, 80,000 and
-    140,000 rays (8, 16, 32 and 64 rays per wave; the rest of each launch is other rays) must come
-    out bit for bit the same."""
+    """Small launches give each wave fewer rays (64 at >= 131,072 rays, then 32, 16, 8, 4 down to
+    about 2,048 waves).  The same 4,096 rays rendered in launches of 4,096, 16,384, 40,000, 80,000
+    and 140,000 rays (4, 8, 16, 32 and 64 rays per wave; the rest of each launch is other rays) must
+    come out bit for bit the same."""
     from diver_amd.renderer import render_rays
     grid = SceneGrid.from_synthetic(scene64, dev)
     dec = Decoder(from_baseline(S.baseline_decoder_layers(32, 32), 32))
@@ -114,7 +114,7 @@ def test_rays_per_wave_does_not_change_results(scene64, cameras, dev, precision)
     rest[idx] = False
     others = torch.nonzero(rest).flatten()
     results = []
-    for total in (4096, 40000, 80000, 140000):
+    for total in (4096, 16384, 40000, 80000, 140000):
         sel = torch.cat([idx, others[:total - 4096]])
         rgb, T, cnt = render_rays(grid, dec, o[sel].contiguous(), d[sel].contiguous(), precision)
         results.append((rgb[:4096], T[:4096], cnt[:4096]))
