@@ -28,6 +28,7 @@ EXPORTS = [
     'diver_server_submit_pinhole', 'diver_server_wait',
     'diver_server_take_completed', 'diver_server_history', 'diver_adam_step_rows',
     'diver_adjoint_decoder_forward', 'diver_adjoint_decoder_backward', 'diver_positional_encoding',
+    'diver_adjoint_weight_grads', 'diver_adjoint_weight_grads_workspace',
 ]
 
 # status codes of include/diver.h
@@ -138,6 +139,8 @@ def _declare(L):
         'diver_adjoint_decoder_backward': [vp, i64, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                                            vp, vp, vp, vp],
         'diver_positional_encoding': [i64, vp, vp, vp],
+        'diver_adjoint_weight_grads': [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i64,
+                                       vp],
     }
     sig.update({
         'diver_server_create': [ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)],
@@ -163,6 +166,8 @@ def _declare(L):
     L.diver_server_history.restype = ctypes.c_int64
     L.diver_decoder_param_count.argtypes = [ctypes.POINTER(DecoderDesc)]
     L.diver_decoder_param_count.restype = ctypes.c_int64
+    L.diver_adjoint_weight_grads_workspace.argtypes = [vp, i64]
+    L.diver_adjoint_weight_grads_workspace.restype = ctypes.c_int64
 
 
 def lib():

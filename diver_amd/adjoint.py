@@ -20,8 +20,9 @@ Backward (for decoders without pre-layers, e.g. the benchmark decoder):
   3. the decoder's backward pass on the matrix cores (``diver_adjoint_decoder_backward``):
         do = dc * c (1 - c);  dr2 = (do W_o) * [y2 > 0];  dr1 = (dr2 W_2) * [y1 > 0];
         dF = dr1 W_c + ds w_s
-     and the weight gradients as batched GEMM reductions over the rows:
-        dW_o = do^T r2;  dW_2 = dr2^T r1;  dW_c = dr1^T F;  dW_v = (per-ray sums of dr1)^T PE(d)
+     and the weight gradients as reductions over the rows in one hand-written pair of launches
+     (``diver_adjoint_weight_grads``, a fixed summation order):
+        dW_o = do^T r2;  dW_2 = dr2^T r1;  dW_c = dr1^T F;  dW_v = dr1^T PE(d)
   4. grid adjoint: ``diver_integrate_backward`` scatters dF through the 8 corner weights.
 The reference obtains the same gradient from torch autograd over stored K x C / B x M
 activations (model/diver.py:141-207, utils/integrator.py:62-113, train.py:102-113).
@@ -53,30 +54,6 @@ def split_canonical(p: torch.Tensor, C: int, H: int):
     parts = torch.split(p, sizes)
     ws, bs, wc, wv, bc, w2, b2, wo, bo = parts
     return (ws, bs, wc.view(H, C), wv.view(H, PE_DIM), bc, w2.view(H, H), b2, wo.view(3, H), bo)
-
-
-def _row_bucket(K: int) -> int:
-    """Row capacity for K rows: the next multiple of max(4096, 2^floor(log2 K) / 8).  Shapes repeat
-    across steps (at most 8 sizes per octave) with at most 12.5 % padding above 32768 rows."""
-    step = max(4096, (1 << (max(K, 1).bit_length() - 1)) // 8)
-    return -(-K // step) * step
-
-
-def _wgrad(a: torch.Tensor, b: torch.Tensor, chunk: int = 4096) -> torch.Tensor:
-    """a.T @ b for tall a [K, m] and b [K, n] (a weight gradient: a reduction over all K rows into
-    at most 64 x 64).  One GEMM of that shape has only a handful of output tiles to spread over the
-    GPU, so the whole chunks of `chunk` rows go through one batched product and a sum over chunks;
-    the remaining rows (fewer than `chunk`) add one plain product."""
-    K = a.shape[0]
-    S = K // chunk
-    if S < 2:
-        return a.T @ b
-    K0 = S * chunk
-    at = a[:K0].reshape(S, chunk, a.shape[1]).transpose(1, 2).contiguous()
-    out = torch.bmm(at, b[:K0].reshape(S, chunk, b.shape[1])).sum(0)
-    if K0 < K:
-        out = out + a[K0:].T @ b[K0:]
-    return out
 
 
 class _FusedRender(Function):
@@ -122,42 +99,22 @@ class _FusedRender(Function):
             gv0 = None if ctx.accumulate is not None else torch.zeros(ctx.vshape, device=dev)
             return gv0, torch.zeros_like(params), *([None] * 9)
         calls.decoder_update(decoder, params, first=decoder.channels + 1)
-        # The row count changes every step; rounded up to a bucket size (_row_bucket), the GEMMs
-        # over the rows repeat their shapes (a new shape costs the GEMM library a fresh solution
-        # lookup, ~0.6 ms a step in all).  Padding rows are zero (T = alpha = 0, F = 0, s = 0,
-        # suffix 0), so every gradient term they produce is exactly zero; only the first K rows
-        # reach the grid scatter.
-        Kp = _row_bucket(K)
-        rows = calls.render_rows(grid, decoder, o, d, ctx.opts, count, row_off, K, capacity=Kp)
+        rows = calls.render_rows(grid, decoder, o, d, ctx.opts, count, row_off, K)
         ray = torch.repeat_interleave(torch.arange(R, device=dev), cnt64, output_size=K)
-        if Kp > K:
-            ray = torch.cat([ray, torch.zeros(Kp - K, dtype=ray.dtype, device=dev)])
-        pin, pout = rows[:K, 0:3], rows[:K, 3:6]
-        F = rows[:, 9:]
+        pin, pout = rows[:, 0:3], rows[:, 3:6]
         # the decoder over the rows on the matrix cores: relu(y1), relu(y2), colour, weight
         r1, r2, c, w = calls.adjoint_decoder_forward(decoder, rows, ray, d)
         # suffix sums within each ray: sum over the later rows of w c, plus the background
         # (one lane per ray, FP64 sums)
-        suffix = calls.ray_suffix(row_off, count, w[:K], c[:K],
+        suffix = calls.ray_suffix(row_off, count, w, c,
                                   rgba[:, 3].contiguous() if ctx.white_back else None)
         # sparsity term: d/ds of g_reg * sum log(1 + s^2 / 0.5) / (B M), applied in the kernel
         sscale = (g_reg / bm) if (ctx.sparsity and g_reg is not None) else None
         dF, do, ds, dr1, dr2 = calls.adjoint_decoder_backward(decoder, rows, ray, K, g.contiguous(),
                                                               suffix, r1, r2, c, w, sscale)
-        # weight gradients: reductions over every row
-        dWo = _wgrad(do, r2)
-        dbo = do.sum(0)
-        dW2 = _wgrad(dr2, r1)
-        db2 = dr2.sum(0)
-        dWc = _wgrad(dr1, F)
-        dbc = dr1.sum(0)
-        # dW_v = sum over rows of dr1 (x) PE(d of the row's ray) = (per-ray sums of dr1)^T PE(d)
-        seg = torch.zeros((R, H), dtype=torch.float32, device=dev).index_add_(0, ray[:K], dr1[:K])
-        dWv = seg.T @ calls.positional_encoding(d)
-        dws = _wgrad(ds[:, None], F)[0]
-        dbs = ds.sum().reshape(1)
-        gp = torch.cat([dws, dbs, dWc.reshape(-1), dWv.reshape(-1), dbc, dW2.reshape(-1), db2,
-                        dWo.reshape(-1), dbo])
+        # weight gradients: reductions over every row, straight into the canonical order
+        # (dw_s, db_s, dW_c, dW_v, db_c, dW_2, db_2, dW_o, db_o)
+        gp = calls.adjoint_weight_grads(decoder, K, rows, ray, d, r1, r2, do, ds, dr1, dr2)
         if ctx.accumulate is not None:
             # scatter straight into the parameter's gradient: no dense temporary of the grid
             leaf = ctx.accumulate

@@ -16,8 +16,8 @@
 //     with the transposed weights as A operands (packed by adj_pack_kernel, in the D order of the
 //     previous product, so each D tile is the next product's B operand in place), and writes dF
 //     (the grid adjoint's input), do, ds, dr1 and dr2 (the weight gradients' operands).
-// The weight gradients are reductions over all rows (dW_o = do^T r2, dW_2 = dr2^T r1, ...) and run
-// as batched GEMMs on the host side.
+// The weight gradients are reductions over all rows (dW_o = do^T r2, dW_2 = dr2^T r1, ...):
+// wgrad_partial_kernel and wgrad_reduce_kernel at the end of this file.
 #include "common.h"
 #include "decoder.h"
 #include "kernels.h"
@@ -305,6 +305,222 @@ hipError_t launch_adjoint(int variant, bool backward, const AdjArgs& p, hipStrea
         case 0: return launch_adj_d<DecB32>(backward, p, s);
         case 1: return launch_adj_d<DecB64>(backward, p, s);
         case 2: return launch_adj_d<DecB64C64>(backward, p, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Weight gradients of the decoder adjoint: reductions over every row of outer products,
+//   dw_s = ds^T F, db_s = sum ds, dW_c = dr1^T F, dW_v = dr1^T PE(d), db_c = sum dr1,
+//   dW_2 = dr2^T r1, db_2 = sum dr2, dW_o = do^T r2, db_o = sum do,
+// written straight into the canonical gradient vector.  Two passes, both in a fixed order (the
+// result does not depend on timing): each workgroup sums a contiguous slab of rows into a partial
+// of its own (wgrad_partial_kernel), then one lane per gradient element adds the workgroups'
+// partials in workgroup order (wgrad_reduce_kernel).
+//
+// Rows are staged through LDS, TK at a time, as two vectors per row:
+//   a = [dr1 (H) | dr2 (H) | do (3) | ds]
+//   b = [F (C) | PE (27) | 1 | 0 ... to CP | r1 (H) | r2 (H) | 1 0 0 0 0 0 0 0]
+// and each lane accumulates one 4 x 8 block ("tile") of sum a b^T in registers.  The tiles cover
+// dr1 x [F | PE | 1] (dW_c, dW_v, db_c), dr2 x r1 (dW_2), dr2 x 1 (db_2) and [do | ds] x (r2, F, 1)
+// (dW_o, dw_s, db_o, db_s); the few products the blocks compute beyond those are never read.
+template <int C, int H>
+struct WgradLayout {
+    static constexpr int CP = (C + kPE + 1 + 7) / 8 * 8;
+    static constexpr int LA = 2 * H + 4;
+    static constexpr int LB = CP + 2 * H + 8;
+    static constexpr int B_R1 = CP, B_R2 = CP + H, B_ONE = CP + 2 * H;
+    static constexpr int N1 = (H / 4) * (CP / 8);
+    static constexpr int N2 = (H / 4) * (H / 8);
+    static constexpr int N3 = H / 4;
+    static constexpr int N4 = H / 8 + C / 8 + 1;
+    static constexpr int NT = N1 + N2 + N3 + N4;
+    static constexpr int THREADS = (NT + 63) / 64 * 64;
+    static constexpr int NP = C + 1 + H * C + H * kPE + H + H * H + H + 3 * H + 3;
+
+    __device__ static void tile(int t, int& a0, int& b0) {
+        if (t < N1) { a0 = 4 * (t / (CP / 8)); b0 = 8 * (t % (CP / 8)); return; }
+        t -= N1;
+        if (t < N2) { a0 = H + 4 * (t / (H / 8)); b0 = B_R1 + 8 * (t % (H / 8)); return; }
+        t -= N2;
+        if (t < N3) { a0 = H + 4 * t; b0 = B_ONE; return; }
+        t -= N3;
+        a0 = 2 * H;
+        b0 = t < H / 8 ? B_R2 + 8 * t : (t < H / 8 + C / 8 ? 8 * (t - H / 8) : B_ONE);
+    }
+    // the tile holding element (ia, ib) of a b^T, for the (ia, ib) the gradients read
+    __device__ static int tile_of(int ia, int ib) {
+        if (ia < H) return (ia / 4) * (CP / 8) + ib / 8;
+        if (ia < 2 * H)
+            return ib == B_ONE ? N1 + N2 + (ia - H) / 4 : N1 + ((ia - H) / 4) * (H / 8) + (ib - B_R1) / 8;
+        if (ib == B_ONE) return N1 + N2 + N3 + H / 8 + C / 8;
+        return ib >= B_R2 ? N1 + N2 + N3 + (ib - B_R2) / 8 : N1 + N2 + N3 + H / 8 + ib / 8;
+    }
+    // canonical gradient element p -> (ia, ib)
+    __device__ static void element(int p, int& ia, int& ib) {
+        if (p < C) { ia = 2 * H + 3; ib = p; return; }                 // w_s
+        if (p == C) { ia = 2 * H + 3; ib = B_ONE; return; }            // b_s
+        int q = p - (C + 1);
+        if (q < H * C) { ia = q / C; ib = q % C; return; }             // W_c
+        q -= H * C;
+        if (q < H * kPE) { ia = q / kPE; ib = C + q % kPE; return; }   // W_v
+        q -= H * kPE;
+        if (q < H) { ia = q; ib = C + kPE; return; }                   // b_c
+        q -= H;
+        if (q < H * H) { ia = H + q / H; ib = B_R1 + q % H; return; }  // W_2
+        q -= H * H;
+        if (q < H) { ia = H + q; ib = B_ONE; return; }                 // b_2
+        q -= H;
+        if (q < 3 * H) { ia = 2 * H + q / H; ib = B_R2 + q % H; return; }  // W_o
+        q -= 3 * H;
+        ia = 2 * H + q; ib = B_ONE;                                    // b_o
+    }
+};
+
+constexpr int kWgradTK = 32;
+
+template <int C, int H>
+__global__ void __launch_bounds__((WgradLayout<C, H>::THREADS))
+wgrad_partial_kernel(WgradArgs p) {
+    using L = WgradLayout<C, H>;
+    constexpr int TK = kWgradTK;
+    __shared__ __align__(16) float sa[TK][L::LA];
+    __shared__ __align__(16) float sb[TK][L::LB];
+    const int tid = threadIdx.x;
+    const int64_t k_begin = (int64_t)blockIdx.x * p.rows_per_group;
+    const int64_t k_end = k_begin + p.rows_per_group < p.n_rows ? k_begin + p.rows_per_group : p.n_rows;
+    // constant columns: the 1 after PE and its zero pad, and the unit block
+    constexpr int NPAD = L::CP - (C + kPE);
+    for (int i = tid; i < TK * NPAD; i += L::THREADS)
+        sb[i / NPAD][C + kPE + i % NPAD] = (i % NPAD) == 0 ? 1.0f : 0.0f;
+    for (int i = tid; i < TK * 8; i += L::THREADS) sb[i / 8][L::B_ONE + i % 8] = (i % 8) == 0 ? 1.0f : 0.0f;
+    int a0 = 0, b0 = 0;
+    const bool active = tid < L::NT;
+    if (active) L::tile(tid, a0, b0);
+    float acc[4][8];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[i][j] = 0.0f;
+    const int stride = 9 + C;
+    for (int64_t k0 = k_begin; k0 < k_end; k0 += TK) {
+        // rows past the slab's end are staged as zeros (a zero a with a finite b adds nothing)
+        for (int i = tid; i < TK * L::LA; i += L::THREADS) {
+            const int r = i / L::LA, c = i % L::LA;
+            const int64_t k = k0 + r;
+            float v = 0.0f;
+            if (k < k_end) {
+                if (c < H) v = p.dr1[k * H + c];
+                else if (c < 2 * H) v = p.dr2[k * H + (c - H)];
+                else if (c < 2 * H + 3) v = p.dout[k * 3 + (c - 2 * H)];
+                else v = p.ds[k];
+            }
+            sa[r][c] = v;
+        }
+        for (int i = tid; i < TK * (C + 2 * H); i += L::THREADS) {
+            const int r = i / (C + 2 * H), c = i % (C + 2 * H);
+            const int64_t k = k0 + r;
+            float v = 0.0f;
+            int col;
+            if (c < C) {
+                col = c;
+                if (k < k_end) v = p.rows[k * stride + 9 + c];
+            } else if (c < C + H) {
+                col = L::B_R1 + (c - C);
+                if (k < k_end) v = p.r1[k * H + (c - C)];
+            } else {
+                col = L::B_R2 + (c - C - H);
+                if (k < k_end) v = p.r2[k * H + (c - C - H)];
+            }
+            sb[r][col] = v;
+        }
+        // PE(d) of the row's ray, element by element as positional_encoding forms it
+        for (int i = tid; i < TK * 12; i += L::THREADS) {
+            const int r = i / 12, q = i % 12, l = q / 3, ax = q % 3;
+            const int64_t k = k0 + r;
+            const float dv = k < k_end ? p.d[p.ray[k] * 3 + ax] : 0.0f;
+            float sv, cv;
+            sincospif((float)(1 << l) * dv, &sv, &cv);
+            sb[r][C + 3 + 6 * l + ax] = sv;
+            sb[r][C + 6 + 6 * l + ax] = cv;
+            if (l == 0) sb[r][C + ax] = dv;
+        }
+        __syncthreads();
+        if (active) {
+#pragma unroll 4
+            for (int r = 0; r < TK; ++r) {
+                const float4 av = *reinterpret_cast<const float4*>(&sa[r][a0]);
+                const float4 bl = *reinterpret_cast<const float4*>(&sb[r][b0]);
+                const float4 bh = *reinterpret_cast<const float4*>(&sb[r][b0 + 4]);
+                const float a[4] = {av.x, av.y, av.z, av.w};
+                const float b[8] = {bl.x, bl.y, bl.z, bl.w, bh.x, bh.y, bh.z, bh.w};
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) acc[i][j] = fmaf(a[i], b[j], acc[i][j]);
+            }
+        }
+        __syncthreads();
+    }
+    if (active) {
+        float* out = p.partial + ((int64_t)blockIdx.x * L::NT + tid) * 32;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 8; ++j) out[i * 8 + j] = acc[i][j];
+    }
+}
+
+template <int C, int H>
+__global__ void __launch_bounds__(256) wgrad_reduce_kernel(const float* __restrict__ partial,
+                                                           int groups, float* __restrict__ grad) {
+    using L = WgradLayout<C, H>;
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= L::NP) return;
+    int ia, ib;
+    L::element(e, ia, ib);
+    const int64_t at = (int64_t)L::tile_of(ia, ib) * 32 + (ia & 3) * 8 + (ib & 7);
+    double s = 0.0;
+    for (int g = 0; g < groups; ++g) s += (double)partial[(int64_t)g * L::NT * 32 + at];
+    grad[e] = (float)s;
+}
+
+int wgrad_tiles(int C, int H) {
+    if (C == 32 && H == 32) return WgradLayout<32, 32>::NT;
+    if (C == 32 && H == 64) return WgradLayout<32, 64>::NT;
+    if (C == 64 && H == 64) return WgradLayout<64, 64>::NT;
+    return 0;
+}
+
+// LDS stages of TK rows per workgroup: the fewest that keep the workgroups within the maximum
+static int64_t wgrad_stages_per_group(int64_t n_rows) {
+    const int64_t tiles = (n_rows + kWgradTK - 1) / kWgradTK;
+    return (tiles + kWgradMaxGroups - 1) / kWgradMaxGroups;
+}
+
+int64_t wgrad_groups(int64_t n_rows) {
+    const int64_t tiles = (n_rows + kWgradTK - 1) / kWgradTK;
+    if (tiles == 0) return 1;
+    const int64_t per = wgrad_stages_per_group(n_rows);
+    return (tiles + per - 1) / per;
+}
+
+template <int C, int H>
+static hipError_t launch_wgrad_ch(WgradArgs p, float* grad, hipStream_t s) {
+    using L = WgradLayout<C, H>;
+    const int64_t groups = wgrad_groups(p.n_rows);
+    p.rows_per_group = wgrad_stages_per_group(p.n_rows) * kWgradTK;
+    hipLaunchKernelGGL((wgrad_partial_kernel<C, H>), dim3((unsigned)groups), dim3(L::THREADS), 0, s, p);
+    hipLaunchKernelGGL((wgrad_reduce_kernel<C, H>), dim3((L::NP + 255) / 256), dim3(256), 0, s,
+                       (const float*)p.partial, (int)groups, grad);
+    return hipGetLastError();
+}
+
+hipError_t launch_weight_grads(int variant, const WgradArgs& p, float* grad, hipStream_t s) {
+    switch (variant) {
+        case 0: return launch_wgrad_ch<32, 32>(p, grad, s);
+        case 1: return launch_wgrad_ch<32, 64>(p, grad, s);
+        case 2: return launch_wgrad_ch<64, 64>(p, grad, s);
         default: return hipErrorInvalidValue;
     }
 }

@@ -863,6 +863,46 @@ int diver_adjoint_decoder_backward(const diver_decoder* dec, int64_t n_rows, int
                       "adjoint_decoder_backward");
 }
 
+int64_t diver_adjoint_weight_grads_workspace(const diver_decoder* dec, int64_t n_rows) {
+    if (!dec || n_rows < 0 || dec->variant < 0 || dec->variant > 2 || dec->desc.n_pre != 0)
+        return -1;
+    const int tiles = diver::wgrad_tiles(dec->desc.channels, dec->desc.hidden);
+    return (int64_t)sizeof(float) * diver::wgrad_groups(n_rows) * tiles * 32;
+}
+
+int diver_adjoint_weight_grads(const diver_decoder* dec, int64_t n_rows, const float* rows,
+                               const int64_t* ray, const float* d, const float* r1, const float* r2,
+                               const float* d_out, const float* d_s, const float* d_r1,
+                               const float* d_r2, float* grad, float* workspace,
+                               int64_t workspace_bytes, diver_stream_t stream) {
+    REQUIRE(dec, "null decoder");
+    REQUIRE(dec->variant >= 0 && dec->variant <= 2 && dec->desc.n_pre == 0,
+            "the weight gradients need a decoder without pre-layers, one post layer and "
+            "channels/hidden 32/32, 32/64 or 64/64");
+    REQUIRE(n_rows >= 0, "n_rows < 0");
+    REQUIRE(grad, "null gradient pointer");
+    const int64_t need = diver_adjoint_weight_grads_workspace(dec, n_rows);
+    REQUIRE(workspace && workspace_bytes >= need,
+            "workspace of %lld bytes needed (diver_adjoint_weight_grads_workspace), %lld given",
+            (long long)need, (long long)workspace_bytes);
+    REQUIRE(n_rows == 0 || (rows && ray && d && r1 && r2 && d_out && d_s && d_r1 && d_r2),
+            "null pointer");
+    diver::WgradArgs p{};
+    p.n_rows = n_rows;
+    p.rows = rows;
+    p.ray = ray;
+    p.d = d;
+    p.r1 = r1;
+    p.r2 = r2;
+    p.dout = d_out;
+    p.ds = d_s;
+    p.dr1 = d_r1;
+    p.dr2 = d_r2;
+    p.partial = workspace;
+    return hip_status(diver::launch_weight_grads(dec->variant, p, grad, S(stream)),
+                      "adjoint_weight_grads");
+}
+
 int diver_positional_encoding(int64_t n, const float* d, float* pe, diver_stream_t stream) {
     REQUIRE(n >= 0, "n < 0");
     if (n == 0) return DIVER_OK;

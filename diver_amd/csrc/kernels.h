@@ -134,6 +134,26 @@ struct AdjArgs {
 int64_t pack_adj_size(int C, int H, int npost);
 hipError_t launch_pack_adj(int C, int H, int npost, const float* W, float* out, hipStream_t s);
 hipError_t launch_adjoint(int variant, bool backward, const AdjArgs& p, hipStream_t s);
+
+// Weight gradients of the decoder adjoint over n_rows rows (variants 0-2: C/H 32/32, 32/64, 64/64).
+struct WgradArgs {
+    int64_t n_rows;
+    int64_t rows_per_group;  // set by the launcher
+    const float* rows;       // [n_rows][9 + C]: F at 9
+    const int64_t* ray;      // [n_rows]
+    const float* d;          // [R][3]
+    const float* r1;         // [n_rows][H]
+    const float* r2;         // [n_rows][H]
+    const float* dout;       // [n_rows][3]
+    const float* ds;         // [n_rows]
+    const float* dr1;        // [n_rows][H]
+    const float* dr2;        // [n_rows][H]
+    float* partial;          // [groups][tiles][32] per-workgroup partial sums
+};
+constexpr int kWgradMaxGroups = 512;
+int wgrad_tiles(int C, int H);                 // tiles per partial (0: unsupported shape)
+int64_t wgrad_groups(int64_t n_rows);          // workgroups of the first pass
+hipError_t launch_weight_grads(int variant, const WgradArgs& p, float* grad, hipStream_t s);
 hipError_t launch_positional_encoding(int64_t R, const float* d, float* pe, hipStream_t s);
 
 // FP32 matrix-core A operands of a decoder's pre-layers (the occupancy kernel's density MLP),

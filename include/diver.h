@@ -337,6 +337,23 @@ int diver_adjoint_decoder_backward(const diver_decoder* dec, int64_t n_rows, int
                                    const float* weight, float* d_feature, float* d_out, float* d_s,
                                    float* d_r1, float* d_r2, diver_stream_t stream);
 
+/* Weight gradients of the decoder adjoint over n_rows rows (decoders of diver_adjoint_decoder_*
+ * with channels/hidden 32/32, 32/64 or 64/64), from the operands the two halves wrote:
+ *   dw_s = d_s^T F, db_s = sum d_s, dW_c = d_r1^T F, dW_v = d_r1^T PE(d[ray]), db_c = sum d_r1,
+ *   dW_2 = d_r2^T r1, db_2 = sum d_r2, dW_o = d_out^T r2, db_o = sum d_out
+ * (F = rows[:, 9:]), written over grad [diver_decoder_param_count] in the canonical parameter
+ * order.  These are the reductions torch autograd forms in the reference's nn.Linear layers of
+ * the colour and density MLPs (model/mlps.py:21-31 through model/diver.py:159-171, train.py:110-113).
+ * Two launches in a fixed summation order (per-workgroup partials in workspace, then a reduction
+ * in workgroup order), so the result does not depend on timing.  workspace: device memory of at
+ * least diver_adjoint_weight_grads_workspace(dec, n_rows) bytes; its contents are overwritten. */
+int64_t diver_adjoint_weight_grads_workspace(const diver_decoder* dec, int64_t n_rows);
+int diver_adjoint_weight_grads(const diver_decoder* dec, int64_t n_rows, const float* rows,
+                               const int64_t* ray, const float* d, const float* r1, const float* r2,
+                               const float* d_out, const float* d_s, const float* d_r1,
+                               const float* d_r2, float* grad, float* workspace,
+                               int64_t workspace_bytes, diver_stream_t stream);
+
 /* The positional encoding PE(d) of n directions, [n][27]: [d, sin(pi d), cos(pi d), ...,
  * sin(8 pi d), cos(8 pi d)] (model/mlps.py:12-18), as the fused kernels evaluate it. */
 int diver_positional_encoding(int64_t n, const float* d, float* pe, diver_stream_t stream);
