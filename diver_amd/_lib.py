@@ -29,6 +29,7 @@ EXPORTS = [
     'diver_server_take_completed', 'diver_server_history', 'diver_adam_step_rows',
     'diver_adjoint_decoder_forward', 'diver_adjoint_decoder_backward', 'diver_positional_encoding',
     'diver_adjoint_weight_grads', 'diver_adjoint_weight_grads_workspace',
+    'diver_adjoint_weight_grads_noalloc',
 ]
 
 # status codes of include/diver.h
@@ -141,6 +142,7 @@ def _declare(L):
         'diver_positional_encoding': [i64, vp, vp, vp],
         'diver_adjoint_weight_grads': [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i64,
                                        vp],
+        'diver_adjoint_weight_grads_noalloc': [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp],
     }
     sig.update({
         'diver_server_create': [ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)],

@@ -315,10 +315,13 @@ def adjoint_decoder_backward(decoder, rows, ray, n_valid, grad_rgb, suffix, r1, 
     return d_feature, d_out, d_s, d_r1, d_r2
 
 
-def adjoint_weight_grads(decoder, n_rows, rows, ray, d, r1, r2, d_out, d_s, d_r1, d_r2):
+def adjoint_weight_grads(decoder, n_rows, rows, ray, d, r1, r2, d_out, d_s, d_r1, d_r2,
+                         no_alloc=False):
     """Weight gradients of the decoder adjoint over the first n_rows rows
-    (diver_adjoint_weight_grads): the canonical parameter gradient (as many elements as the decoder's parameters) of
-    dW_c, dW_v, dW_2, dW_o, dw_s and the biases, summed in a fixed order."""
+    (diver_adjoint_weight_grads): the canonical parameter gradient (as many elements as the
+    decoder's parameters) of dW_c, dW_v, dW_2, dW_o, dw_s and the biases, summed in a fixed order.
+    With no_alloc the library's own partial-sum array is used (diver_adjoint_weight_grads_noalloc;
+    calls on one device must not overlap) instead of a workspace allocated here."""
     tensors = [rows, ray, d, r1, r2, d_out, d_s, d_r1, d_r2]
     dev = _dev(*tensors)
     n = int(n_rows)
@@ -331,13 +334,18 @@ def adjoint_weight_grads(decoder, n_rows, rows, ray, d, r1, r2, d_out, d_s, d_r1
     if ray.dtype != torch.int64 or not ray.is_contiguous() or ray.shape[0] < n:
         raise TypeError('ray must be a contiguous int64 tensor of at least n_rows entries')
     d = _f32(d, 'd')
+    grad = torch.empty((int(decoder.canon.params.size),), dtype=torch.float32, device=dev)
+    if no_alloc:
+        _lib.call('diver_adjoint_weight_grads_noalloc', decoder.handle, n, rows.data_ptr(),
+                  ray.data_ptr(), d.data_ptr(), r1.data_ptr(), r2.data_ptr(), d_out.data_ptr(),
+                  d_s.data_ptr(), d_r1.data_ptr(), d_r2.data_ptr(), grad.data_ptr(), _s(rows))
+        return grad
     L = _lib.lib()
     need = L.diver_adjoint_weight_grads_workspace(decoder.handle, n)
     if need < 0:
         raise ValueError('the weight gradients need a decoder without pre-layers and '
                          'channels/hidden 32/32, 32/64 or 64/64')
     ws = torch.empty((max(need, 4) // 4,), dtype=torch.float32, device=dev)
-    grad = torch.empty((int(decoder.canon.params.size),), dtype=torch.float32, device=dev)
     _lib.call('diver_adjoint_weight_grads', decoder.handle, n, rows.data_ptr(), ray.data_ptr(),
               d.data_ptr(), r1.data_ptr(), r2.data_ptr(), d_out.data_ptr(), d_s.data_ptr(),
               d_r1.data_ptr(), d_r2.data_ptr(), grad.data_ptr(), ws.data_ptr(), ws.numel() * 4,

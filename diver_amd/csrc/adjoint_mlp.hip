@@ -18,6 +18,8 @@
 //     (the grid adjoint's input), do, ds, dr1 and dr2 (the weight gradients' operands).
 // The weight gradients are reductions over all rows (dW_o = do^T r2, dW_2 = dr2^T r1, ...):
 // wgrad_partial_kernel and wgrad_reduce_kernel at the end of this file.
+#include <mutex>
+
 #include "common.h"
 #include "decoder.h"
 #include "kernels.h"
@@ -514,6 +516,39 @@ static hipError_t launch_wgrad_ch(WgradArgs p, float* grad, hipStream_t s) {
     hipLaunchKernelGGL((wgrad_reduce_kernel<C, H>), dim3((L::NP + 255) / 256), dim3(256), 0, s,
                        (const float*)p.partial, (int)groups, grad);
     return hipGetLastError();
+}
+
+// Allocation-free form of the weight gradients: the per-workgroup partials live in this file-scope
+// device array instead of a caller's workspace.  Its size covers the largest partial any row count
+// needs (the first pass never uses more than kWgradMaxGroups workgroups), 23.1 MB per device, held
+// until the process ends.  Host code never passes `scratch` itself: the array's device address is
+// looked up once per device (hipGetSymbolAddress, with that device current) and kept here.
+constexpr int64_t kWgradScratchFloats = (int64_t)kWgradMaxGroups * WgradLayout<64, 64>::NT * 32;
+static_assert(WgradLayout<64, 64>::NT >= WgradLayout<32, 64>::NT &&
+              WgradLayout<64, 64>::NT >= WgradLayout<32, 32>::NT, "largest partial");
+static __device__ __align__(256) float scratch[kWgradScratchFloats];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+float* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+hipError_t wgrad_scratch(float** ptr, int64_t* bytes) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* addr = nullptr;
+        e = hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[dev] = static_cast<float*>(addr);
+    }
+    *ptr = g_scratch_ptr[dev];
+    *bytes = kWgradScratchFloats * (int64_t)sizeof(float);
+    return hipSuccess;
 }
 
 hipError_t launch_weight_grads(int variant, const WgradArgs& p, float* grad, hipStream_t s) {

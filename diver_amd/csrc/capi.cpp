@@ -903,6 +903,27 @@ int diver_adjoint_weight_grads(const diver_decoder* dec, int64_t n_rows, const f
                       "adjoint_weight_grads");
 }
 
+int diver_adjoint_weight_grads_noalloc(const diver_decoder* dec, int64_t n_rows, const float* rows,
+                                       const int64_t* ray, const float* d, const float* r1,
+                                       const float* r2, const float* d_out, const float* d_s,
+                                       const float* d_r1, const float* d_r2, float* grad,
+                                       diver_stream_t stream) {
+    REQUIRE(dec, "null decoder");
+    REQUIRE(n_rows >= 0, "n_rows < 0");
+    const int64_t need = diver_adjoint_weight_grads_workspace(dec, n_rows);
+    REQUIRE(need >= 0,
+            "the weight gradients need a decoder without pre-layers, one post layer and "
+            "channels/hidden 32/32, 32/64 or 64/64");
+    float* ws = nullptr;
+    int64_t cap = 0;
+    const hipError_t e = diver::wgrad_scratch(&ws, &cap);
+    if (e != hipSuccess) return hip_status(e, "adjoint_weight_grads_noalloc: scratch address");
+    REQUIRE(need <= cap, "the partial sums need %lld bytes, above the %lld-byte scratch array",
+            (long long)need, (long long)cap);
+    return diver_adjoint_weight_grads(dec, n_rows, rows, ray, d, r1, r2, d_out, d_s, d_r1, d_r2,
+                                      grad, ws, cap, stream);
+}
+
 int diver_positional_encoding(int64_t n, const float* d, float* pe, diver_stream_t stream) {
     REQUIRE(n >= 0, "n < 0");
     if (n == 0) return DIVER_OK;

@@ -154,6 +154,9 @@ constexpr int kWgradMaxGroups = 512;
 int wgrad_tiles(int C, int H);                 // tiles per partial (0: unsupported shape)
 int64_t wgrad_groups(int64_t n_rows);          // workgroups of the first pass
 hipError_t launch_weight_grads(int variant, const WgradArgs& p, float* grad, hipStream_t s);
+// Device address and size of the file-scope partial array of the allocation-free weight
+// gradients on the current device (looked up on the first call per device).
+hipError_t wgrad_scratch(float** ptr, int64_t* bytes);
 hipError_t launch_positional_encoding(int64_t R, const float* d, float* pe, hipStream_t s);
 
 // FP32 matrix-core A operands of a decoder's pre-layers (the occupancy kernel's density MLP),

@@ -354,6 +354,19 @@ int diver_adjoint_weight_grads(const diver_decoder* dec, int64_t n_rows, const f
                                const float* d_r2, float* grad, float* workspace,
                                int64_t workspace_bytes, diver_stream_t stream);
 
+/* Allocation-free form of diver_adjoint_weight_grads, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight training loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  The per-workgroup partials live in a file-scope device array
+ * of the library (23.1 MB per device, held until the process ends; every decoder shape and row
+ * count fits it); the call allocates nothing and otherwise behaves as diver_adjoint_weight_grads,
+ * bit for bit.  Every call on one device shares that device's array: calls that overlap on one
+ * device (two streams, two threads) race and must be ordered by the caller. */
+int diver_adjoint_weight_grads_noalloc(const diver_decoder* dec, int64_t n_rows, const float* rows,
+                                       const int64_t* ray, const float* d, const float* r1,
+                                       const float* r2, const float* d_out, const float* d_s,
+                                       const float* d_r1, const float* d_r2, float* grad,
+                                       diver_stream_t stream);
+
 /* The positional encoding PE(d) of n directions, [n][27]: [d, sin(pi d), cos(pi d), ...,
  * sin(8 pi d), cos(8 pi d)] (model/mlps.py:12-18), as the fused kernels evaluate it. */
 int diver_positional_encoding(int64_t n, const float* d, float* pe, diver_stream_t stream);
