@@ -29,7 +29,7 @@ EXPORTS = [
     'diver_server_take_completed', 'diver_server_history', 'diver_adam_step_rows',
     'diver_adjoint_decoder_forward', 'diver_adjoint_decoder_backward', 'diver_positional_encoding',
     'diver_adjoint_weight_grads', 'diver_adjoint_weight_grads_workspace',
-    'diver_adjoint_weight_grads_noalloc',
+    'diver_adjoint_weight_grads_noalloc', 'diver_render_rays_segments', 'diver_render_rows_segments',
 ]
 
 # status codes of include/diver.h
@@ -143,6 +143,10 @@ def _declare(L):
         'diver_adjoint_weight_grads': [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i64,
                                        vp],
         'diver_adjoint_weight_grads_noalloc': [vp, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp],
+        'diver_render_rays_segments': [ctypes.POINTER(Grid), vp, ctypes.POINTER(RenderOpts), i64,
+                                       vp, vp, vp, vp, vp, vp, i32, vp],
+        'diver_render_rows_segments': [ctypes.POINTER(Grid), vp, ctypes.POINTER(RenderOpts), i64,
+                                       vp, vp, vp, i32, vp, vp],
     }
     sig.update({
         'diver_server_create': [ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)],

@@ -9,8 +9,9 @@ of the batch, train.py:84-85,110; the zero padding adds nothing): the kernel sum
 log(1 + 2 s^2) per ray, and its adjoint adds g / (B M) * 4 s / (1 + 2 s^2) to each interval's ds.
 
 Backward (for decoders without pre-layers, e.g. the benchmark decoder):
-  1. re-walk every ray and recompute each composited interval's integral, density and
-     transmittance (``diver_render_rows``);
+  1. each composited interval's integral, density and transmittance, from the entry and exit the
+     forward recorded (``diver_render_rays_segments`` / ``diver_render_rows_segments``) or, for a
+     batch too large to record, by re-walking every ray (``diver_render_rows``);
   2. the decoder over the rows on the matrix cores (``diver_adjoint_decoder_forward``: relu(y1),
      relu(y2), colour c and weight w = T a per row), then the compositing adjoint, per interval k
      of a ray with C = sum_i T_i a_i c_i + bg T_n:
@@ -56,6 +57,11 @@ def split_canonical(p: torch.Tensor, C: int, H: int):
     return (ws, bs, wc.view(H, C), wv.view(H, PE_DIM), bc, w2.view(H, H), b2, wo.view(3, H), bo)
 
 
+# Device memory the forward may spend on recorded segments (24 bytes per ray and step bound); a
+# larger batch re-walks its rays in the backward instead.
+SEGMENT_BYTES = 2 << 30
+
+
 class _FusedRender(Function):
     @staticmethod
     def forward(ctx, voxels, params, grid, decoder, o, d, t_stop, white_back, traversal,
@@ -64,14 +70,27 @@ class _FusedRender(Function):
         # part is pushed and the kernel keeps its channel-0 density path
         calls.decoder_update(decoder, params, first=decoder.channels + 1)
         opts = calls.render_opts('fp32', traversal, t_stop, 0.0)
-        if sparsity:
+        # With a gradient to come, the offline walk records each interval's entry and exit
+        # ([R, cap, 6], cap = the walk's step bound 3N + 8, so every interval fits), and the
+        # backward builds its rows from them instead of walking every ray again.
+        cap = 3 * grid.n + 8
+        segments = None
+        if (any(ctx.needs_input_grad[:2]) and traversal == 'offline'
+                and getattr(grid, 'blocks', None) is not None
+                and getattr(grid, 'bricks', None) is not None
+                and o.shape[0] * cap * 24 <= SEGMENT_BYTES):
+            rgba, count, sreg, segments = calls.render_rays_segments(grid, decoder, o, d, opts, cap,
+                                                                     sparsity=sparsity)
+        elif sparsity:
             rgba, count, sreg = calls.render_rays_sparsity(grid, decoder, o, d, opts)
+        else:
+            rgba, count = calls.render_rays(grid, decoder, o, d, opts)
+        if sparsity:
             # B x M with M the batch's largest interval count, kept on the device (no sync)
             bm = (count.max().to(torch.float32) * o.shape[0]).clamp_min(1.0)
         else:
-            rgba, count = calls.render_rays(grid, decoder, o, d, opts)
             bm = torch.ones((), device=o.device)
-        ctx.save_for_backward(params, o, d, rgba, count, bm)
+        ctx.save_for_backward(params, o, d, rgba, count, bm, segments)
         ctx.grid, ctx.decoder, ctx.opts = grid, decoder, opts
         ctx.white_back = white_back
         ctx.vshape = tuple(voxels.shape)
@@ -85,7 +104,7 @@ class _FusedRender(Function):
 
     @staticmethod
     def backward(ctx, g, g_reg=None):
-        params, o, d, rgba, count, bm = ctx.saved_tensors
+        params, o, d, rgba, count, bm, segments = ctx.saved_tensors
         if g is None:
             g = torch.zeros_like(rgba[:, :3])
         grid, decoder = ctx.grid, ctx.decoder
@@ -99,7 +118,10 @@ class _FusedRender(Function):
             gv0 = None if ctx.accumulate is not None else torch.zeros(ctx.vshape, device=dev)
             return gv0, torch.zeros_like(params), *([None] * 9)
         calls.decoder_update(decoder, params, first=decoder.channels + 1)
-        rows = calls.render_rows(grid, decoder, o, d, ctx.opts, count, row_off, K)
+        if segments is not None:
+            rows = calls.render_rows_segments(grid, decoder, ctx.opts, count, row_off, K, segments)
+        else:
+            rows = calls.render_rows(grid, decoder, o, d, ctx.opts, count, row_off, K)
         ray = torch.repeat_interleave(torch.arange(R, device=dev), cnt64, output_size=K)
         pin, pout = rows[:, 0:3], rows[:, 3:6]
         # the decoder over the rows on the matrix cores: relu(y1), relu(y2), colour, weight

@@ -425,6 +425,44 @@ def render_rays_sparsity(grid, decoder, o, d, opts):
     return rgba, count, sparsity
 
 
+def render_rays_segments(grid, decoder, o, d, opts, seg_cap, sparsity=True):
+    """The training forward with each evaluated interval's (entry, exit) recorded
+    (diver_render_rays_segments): (rgba [R, 4], count [R], sparsity [R] or None,
+    segments [R, seg_cap, 6])."""
+    dev = _dev(grid.voxels, grid.mask, o, d)
+    o, d = _f32(o, 'o'), _f32(d, 'd')
+    R = o.shape[0]
+    rgba = torch.empty((R, 4), dtype=torch.float32, device=dev)
+    count = torch.empty(R, dtype=torch.int32, device=dev)
+    sreg = torch.empty(R, dtype=torch.float32, device=dev) if sparsity else None
+    segments = torch.empty((R, int(seg_cap), 6), dtype=torch.float32, device=dev)
+    g = _grid_of(grid)
+    _lib.call('diver_render_rays_segments', ctypes.byref(g), decoder.handle, ctypes.byref(opts),
+              R, o.data_ptr(), d.data_ptr(), rgba.data_ptr(), count.data_ptr(), _lib.ptr(sreg),
+              segments.data_ptr(), int(seg_cap), _s(o))
+    return rgba, count, sreg, segments
+
+
+def render_rows_segments(grid, decoder, opts, count, row_offset, n_rows, segments):
+    """diver_render_rows from recorded segments [R, seg_cap, 6] (every count <= seg_cap):
+    rows [n_rows, 9 + C], the same as render_rows' re-walk."""
+    dev = _dev(grid.voxels, grid.mask, count, row_offset, segments)
+    count = _i32(count, 'count')
+    if row_offset.dtype != torch.int64:
+        raise TypeError('row_offset must be int64')
+    row_offset = row_offset.contiguous()
+    segments = _f32(segments, 'segments')
+    R, cap = segments.shape[0], segments.shape[1]
+    if count.shape[0] != R:
+        raise ValueError('count and segments disagree on the ray count')
+    rows = torch.empty((int(n_rows), 9 + grid.channels), dtype=torch.float32, device=dev)
+    g = _grid_of(grid)
+    _lib.call('diver_render_rows_segments', ctypes.byref(g), decoder.handle, ctypes.byref(opts), R,
+              count.data_ptr(), row_offset.data_ptr(), segments.data_ptr(), int(cap),
+              rows.data_ptr(), _s(count))
+    return rows
+
+
 def corner_keys(n1, coord_in, coord_out):
     """int64 [K, 8]: keys z n1^2 + y n1 + x of the 8 corners of each row's integral."""
     dev = _dev(coord_in, coord_out)

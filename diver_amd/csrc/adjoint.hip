@@ -272,6 +272,10 @@ hipError_t launch_ray_suffix(int64_t R, const int64_t* off, const int32_t* cnt, 
 }
 
 template <class D>
+static hipError_t launch_rows_fill(const RenderArgs& a, bool dch0, const int32_t* count,
+                                   const int64_t* off, float* rows, hipStream_t s);
+
+template <class D>
 static hipError_t launch_rows_d(const RenderArgs& a, bool dch0, const int32_t* count,
                                 const int64_t* off, float* rows, hipStream_t s) {
     const int rpw = rays_per_wave(a.n_rays);
@@ -290,6 +294,62 @@ static hipError_t launch_rows_d(const RenderArgs& a, bool dch0, const int32_t* c
         hipLaunchKernelGGL((rows_kernel<D, false>), grid, block, 0, s, a, count, off, rows, rpw);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    return launch_rows_fill<D>(a, dch0, count, off, rows, s);
+}
+
+// Entry and exit of each row copied from the forward's segments [R][cap][6]: 8 lanes per ray.
+template <class D>
+__global__ void __launch_bounds__(256) rows_segments_kernel(int64_t R, const int32_t* __restrict__ count,
+                                                            const int64_t* __restrict__ row_off,
+                                                            const float* __restrict__ seg, int cap,
+                                                            float* __restrict__ rows) {
+    constexpr int RW = 9 + D::C;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t r = t >> 3;
+    if (r >= R) return;
+    const int m = count[r] < cap ? count[r] : cap;
+    const float2* in = reinterpret_cast<const float2*>(seg + r * (int64_t)cap * 6);
+    float* out = rows + row_off[r] * RW;
+    for (int k = (int)(t & 7); k < m; k += 8) {
+        const float2 a0 = in[3 * k], a1 = in[3 * k + 1], a2 = in[3 * k + 2];
+        float* o = out + (int64_t)k * RW;
+        o[0] = a0.x;
+        o[1] = a0.y;
+        o[2] = a1.x;
+        o[3] = a1.y;
+        o[4] = a2.x;
+        o[5] = a2.y;
+    }
+}
+
+template <class D>
+static hipError_t launch_rows_segments_d(const RenderArgs& a, bool dch0, const int32_t* count,
+                                         const int64_t* off, const float* seg, int cap, float* rows,
+                                         hipStream_t s) {
+    hipLaunchKernelGGL((rows_segments_kernel<D>), dim3((unsigned)((a.n_rays * 8 + 255) / 256)), dim3(256),
+                       0, s, a.n_rays, count, off, seg, cap, rows);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_rows_fill<D>(a, dch0, count, off, rows, s);
+}
+
+hipError_t launch_rows_from_segments(int variant, const RenderArgs& a, bool dch0, const int32_t* count,
+                                     const int64_t* off, const float* segments, int seg_cap,
+                                     float* rows, hipStream_t s) {
+    if (a.n_rays == 0) return hipSuccess;
+    switch (variant) {
+        case 0: return launch_rows_segments_d<DecB32>(a, dch0, count, off, segments, seg_cap, rows, s);
+        case 1: return launch_rows_segments_d<DecB64>(a, dch0, count, off, segments, seg_cap, rows, s);
+        case 2: return launch_rows_segments_d<DecB64C64>(a, dch0, count, off, segments, seg_cap, rows, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+// The rows' features, s, sigma, alpha and transmittance once their entry and exit are written.
+template <class D>
+static hipError_t launch_rows_fill(const RenderArgs& a, bool dch0, const int32_t* count,
+                                   const int64_t* off, float* rows, hipStream_t s) {
+    hipError_t e;
     // 2048 x 256 lanes: 65,536 rows per pass over the grid, enough to fill the GPU
     const dim3 rgrid((unsigned)((a.n_rays + 255) / 256));
     if (dch0) {

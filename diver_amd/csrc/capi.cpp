@@ -680,6 +680,49 @@ int diver_render_rays_sparsity(const diver_grid* grid, const diver_decoder* dec,
                       "render_rays_sparsity");
 }
 
+int diver_render_rays_segments(const diver_grid* grid, const diver_decoder* dec,
+                               const diver_render_opts* opts, int64_t n_rays, const float* o,
+                               const float* d, float* rgba, int32_t* count, float* sparsity,
+                               float* segments, int seg_cap, diver_stream_t stream) {
+    diver::RenderArgs a{};
+    int st = render_common(grid, dec, opts, a);
+    if (st != DIVER_OK) return st;
+    REQUIRE(n_rays >= 0, "n_rays < 0");
+    REQUIRE(n_rays == 0 || (o && d && rgba && count && segments), "null pointer");
+    REQUIRE(seg_cap >= 1, "seg_cap must be positive");
+    REQUIRE(opts->precision == DIVER_FP32 && opts->traversal == DIVER_WALK_OFFLINE &&
+                opts->alpha_skip == 0.0f,
+            "segments are recorded by the FP32 offline walk with alpha_skip == 0 (the training "
+            "forward)");
+    a.n_rays = n_rays;
+    a.o = o;
+    a.d = d;
+    a.cam.width = 0;
+    a.rgba = rgba;
+    a.count = count;
+    if (!tiles_ok(dec, a) || !dec->density_ch0)
+        return fail(DIVER_EUNSUPPORTED,
+                    "segments need the tile kernel (block and brick bits, a decoder without "
+                    "pre-layers) and the channel-0 density (w_s = e_0, b_s = 0)");
+    diver::BatchArgs p{};
+    p.g = a;
+    p.n_req = 1;
+    p.req[0].cam = a.cam;
+    p.req[0].n_rays = n_rays;
+    p.req[0].o = o;
+    p.req[0].d = d;
+    p.req[0].rgba = rgba;
+    p.req[0].count = count;
+    p.req[0].sparsity = sparsity;
+    p.total = n_rays;
+    p.sparsity = sparsity ? 1 : 0;
+    p.segments = segments;
+    p.seg_cap = seg_cap;
+    return hip_status(diver::launch_render_tiles(dec->variant, p, opts->precision,
+                                                 dec->density_ch0, S(stream)),
+                      "render_rays_segments");
+}
+
 int diver_alpha_map(const diver_grid* grid, const diver_decoder* dec, int64_t n_rays,
                     const float* o, const float* d, int pad, float* alpha_map,
                     diver_stream_t stream) {
@@ -793,6 +836,26 @@ int diver_render_rows(const diver_grid* grid, const diver_decoder* dec,
     a.d = d;
     return hip_status(diver::launch_rows(dec->variant, a, dec->density_ch0, count, row_offset, rows, S(stream)),
                       "render_rows");
+}
+
+int diver_render_rows_segments(const diver_grid* grid, const diver_decoder* dec,
+                               const diver_render_opts* opts, int64_t n_rays, const int32_t* count,
+                               const int64_t* row_offset, const float* segments, int seg_cap,
+                               float* rows, diver_stream_t stream) {
+    diver::RenderArgs a{};
+    int st = render_common(grid, dec, opts, a);
+    if (st != DIVER_OK) return st;
+    if (dec->desc.n_pre != 0)
+        return fail(DIVER_EUNSUPPORTED, "rows are written for decoders without pre-layers");
+    REQUIRE(opts->alpha_skip == 0.0f, "the adjoint needs alpha_skip == 0");
+    REQUIRE(n_rays >= 0, "n_rays < 0");
+    REQUIRE(seg_cap >= 1, "seg_cap must be positive");
+    REQUIRE(n_rays == 0 || (count && row_offset && segments && rows), "null pointer");
+    a.n_rays = n_rays;
+    return hip_status(diver::launch_rows_from_segments(dec->variant, a, dec->density_ch0, count,
+                                                       row_offset, segments, seg_cap, rows,
+                                                       S(stream)),
+                      "render_rows_segments");
 }
 
 int diver_adjoint_decoder_forward(const diver_decoder* dec, int64_t n_rows, const float* rows,

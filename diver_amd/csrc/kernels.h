@@ -62,6 +62,9 @@ struct BatchArgs {
     int rays_per_wave;             // ids per wavefront (64, or fewer for small launches; set by
                                    // the launcher)
     int sparsity;                  // some request has a sparsity output
+    float* segments;               // [rays][seg_cap][6] entry and exit of each evaluated interval
+                                   // (single-request FP32 offline launches; null = not recorded)
+    int seg_cap;
 };
 
 // Rays per wavefront for a launch of `total` rays: 64 when the launch fills the GPU, else halved
@@ -190,6 +193,10 @@ hipError_t launch_walk(const RenderArgs& a, int max_n, float* isect, int32_t* nu
 hipError_t launch_tail_bits(const uint32_t* blocks, int n, uint32_t* tail, hipStream_t s);
 hipError_t launch_ray_suffix(int64_t R, const int64_t* off, const int32_t* cnt, const float* w,
                              const float* c, const float* bg, float* out, hipStream_t s);
+// The rows from the forward's recorded segments instead of a re-walk (the same rows, bit for bit).
+hipError_t launch_rows_from_segments(int variant, const RenderArgs& a, bool dch0, const int32_t* count,
+                                     const int64_t* off, const float* segments, int seg_cap,
+                                     float* rows, hipStream_t s);
 hipError_t launch_rows(int variant, const RenderArgs& a, bool dch0, const int32_t* count,
                        const int64_t* off, float* rows, hipStream_t s);
 hipError_t launch_camera_rays(const CameraDesc& cam, float* o, float* d, hipStream_t s);

@@ -290,9 +290,10 @@ struct TileLane {
 // live ray of the wave takes the reciprocal-division step (the usual case); such a walk needs no
 // step bound, since each step moves one axis to its next integer plane ahead and the walk leaves
 // the grid after at most 3(N+1) steps.
-template <class D, bool RT, bool DCH0, bool AF, bool B32>
+// SEG: also record each evaluated interval's entry and exit at seg_base + its index along the ray.
+template <class D, bool RT, bool DCH0, bool AF, bool B32, bool SEG>
 __device__ __forceinline__ void phase_a(TileLane& L, const BatchArgs& p, const uint32_t* blocks,
-                                        float* queue, int lane) {
+                                        float* queue, int lane, int64_t seg_base) {
     constexpr int C = D::C;
     const RenderArgs& a = p.g;
     const int n = a.n;
@@ -344,6 +345,15 @@ __device__ __forceinline__ void phase_a(TileLane& L, const BatchArgs& p, const u
                         density_alpha<D>(s, sigma, alpha);
                         L.used += 1;
                         if (p.sparsity) L.reg += sparsity_term(s);
+                        if constexpr (SEG) {
+                            if (L.used <= p.seg_cap) {
+                                float2* sp = reinterpret_cast<float2*>(p.segments +
+                                                                       (seg_base + L.used - 1) * 6);
+                                sp[0] = make_float2(px, py);
+                                sp[1] = make_float2(pz, qx);
+                                sp[2] = make_float2(qy, qz);
+                            }
+                        }
                         if (alpha >= a.alpha_skip) {
                             wgt = L.T * alpha;
                             push = true;
@@ -378,7 +388,7 @@ constexpr int tile_waves_per_simd() {
     return D::C > 32 && PREC != kPrecFP32 && DCH0 ? 3 : 2;
 }
 
-template <class D, int PREC, bool RT, bool DCH0, bool B32>
+template <class D, int PREC, bool RT, bool DCH0, bool B32, bool SEG = false>
 __global__ void __launch_bounds__(64 * kWavesPerGroup, (tile_waves_per_simd<D, PREC, DCH0>())) render_tile_kernel(BatchArgs p) {
     static_assert(D::NPRE == 0, "the tile kernel needs density from the integrated feature");
     constexpr int C = D::C, H = D::H, Z = D::Z;
@@ -449,13 +459,14 @@ __global__ void __launch_bounds__(64 * kWavesPerGroup, (tile_waves_per_simd<D, P
     }
     // every live ray's direction takes the reciprocal-division step (wave-uniform)
     const bool all_fast = !wave_any(L.ray.alive && !L.ray.fast);
+    const int64_t seg_base = SEG ? out_idx * (int64_t)p.seg_cap : 0;
 
     while (true) {
         // ---------------- phase A: walk, density, transmittance, enqueue ------------------------
         if (all_fast)
-            phase_a<D, RT, DCH0, true, B32>(L, p, blocks, queue, lane);
+            phase_a<D, RT, DCH0, true, B32, SEG>(L, p, blocks, queue, lane, seg_base);
         else
-            phase_a<D, RT, DCH0, false, B32>(L, p, blocks, queue, lane);
+            phase_a<D, RT, DCH0, false, B32, SEG>(L, p, blocks, queue, lane, seg_base);
         const int qn = L.qn;
         if (qn == 0) break;  // nothing queued and every lane done
         // a wave still walking after kLongWalk iterations shades and walks at the highest issue
@@ -528,6 +539,16 @@ hipError_t launch_tiles3(const BatchArgs& p, bool dch0, hipStream_t s) {
     const int64_t per_group = (int64_t)p.rays_per_wave * kWavesPerGroup;
     dim3 grid((unsigned)((p.total + per_group - 1) / per_group)), block(threads);
     size_t lds = render_tile_lds_bytes(p.g.n, D::H, PREC);
+    if (p.segments) {
+        // segments are recorded by the training forward only: FP32, offline walk, channel-0 density
+        if constexpr (PREC == kPrecFP32 && !RT) {
+            if (!dch0 || p.n_req != 1) return hipErrorInvalidValue;
+            hipLaunchKernelGGL((render_tile_kernel<D, PREC, RT, true, B32, true>), grid, block, lds, s, p);
+            return hipGetLastError();
+        } else {
+            return hipErrorInvalidValue;
+        }
+    }
     if (dch0)
         hipLaunchKernelGGL((render_tile_kernel<D, PREC, RT, true, B32>), grid, block, lds, s, p);
     else

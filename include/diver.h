@@ -299,6 +299,26 @@ int diver_render_rows(const diver_grid* grid, const diver_decoder* dec,
                       const float* d, const int32_t* count, const int64_t* row_offset,
                       float* rows, diver_stream_t stream);
 
+/* The training forward with its intervals recorded: diver_render_rays_sparsity (sparsity may be
+ * NULL here) that also writes, for the k-th interval ray r evaluates (k < seg_cap), its entry and
+ * exit in grid units to segments [n_rays][seg_cap][6] (p_in, p_out).  FP32 offline walk,
+ * alpha_skip == 0, block and brick bits, a decoder without pre-layers whose density is channel 0
+ * (w_s = e_0, b_s = 0).  A ray evaluates at most 3 (N + 1) intervals, so seg_cap = 3 (N + 1)
+ * records every one.  With diver_render_rows_segments this replaces the re-walk of
+ * diver_render_rows, which recomputes the intervals the reference's autograd keeps from
+ * masked_intersect (model/diver.py:87-93). */
+int diver_render_rays_segments(const diver_grid* grid, const diver_decoder* dec,
+                               const diver_render_opts* opts, int64_t n_rays, const float* o,
+                               const float* d, float* rgba, int32_t* count, float* sparsity,
+                               float* segments, int seg_cap, diver_stream_t stream);
+
+/* diver_render_rows from the segments of diver_render_rays_segments instead of a re-walk: the same
+ * rows bit for bit, provided every count <= seg_cap (rows past seg_cap of a ray are not written). */
+int diver_render_rows_segments(const diver_grid* grid, const diver_decoder* dec,
+                               const diver_render_opts* opts, int64_t n_rays, const int32_t* count,
+                               const int64_t* row_offset, const float* segments, int seg_cap,
+                               float* rows, diver_stream_t stream);
+
 /* Per-ray suffix sums for the compositing adjoint over the rows of diver_render_rows: for row k of
  * ray r (rows row_offset[r] .. row_offset[r] + count[r] - 1),
  *   suffix[k][ch] = sum over the ray's rows j > k of weight[j] * color[j][ch]  (+ background[r])

@@ -247,3 +247,35 @@ def test_render_rows_match_byte_mask_path(dev, traversal, t_stop):
     t_after = tb * torch.exp(-torch.nn.functional.softplus(s))
     t_fwd = rgba[cnt64 > 0, 3].double()
     assert float((t_after - t_fwd).abs().max()) <= 1e-5
+
+
+@pytest.mark.parametrize('t_stop,hidden', [(0.0, 64), (1e-4, 64), (0.0, 32)])
+def test_recorded_segments_match_rewalk(dev, t_stop, hidden):
+    """The training forward that records each interval's entry and exit
+    (diver_render_rays_segments) returns the same rgba, counts and sparsity sums as the plain
+    forward, bit for bit, and the rows built from its segments (diver_render_rows_segments) equal
+    the re-walk's rows (diver_render_rows) bit for bit; the recorded segments past each ray's count
+    are never read."""
+    from diver_amd import calls
+    sc, _gt, o, d, layers = _setup(dev, n_rays=4099, hidden=hidden, seed=9)
+    grid = SceneGrid.from_synthetic(sc, dev)
+    model = FusedRenderer(grid, BenchmarkDecoder(layers, 32).to(dev), traversal='offline')
+    og, dg = torch.from_numpy(o).to(dev), torch.from_numpy(d).to(dev)
+    opts = calls.render_opts('fp32', 'offline', t_stop, 0.0)
+    cap = 3 * grid.n + 8
+    print(f'segments t_stop={t_stop} hidden={hidden}', flush=True)
+    rgba0, count0, sreg0 = calls.render_rays_sparsity(grid, model.handle, og, dg, opts)
+    rgba, count, sreg, seg = calls.render_rays_segments(grid, model.handle, og, dg, opts, cap)
+    assert torch.equal(rgba, rgba0) and torch.equal(count, count0) and torch.equal(sreg, sreg0)
+    _, count_ns, sreg_ns, seg_ns = calls.render_rays_segments(grid, model.handle, og, dg, opts, cap,
+                                                              sparsity=False)
+    assert sreg_ns is None and torch.equal(count_ns, count0)
+    cnt64 = count.to(torch.int64)
+    assert int(cnt64.max()) <= cap
+    off = torch.cumsum(cnt64, 0) - cnt64
+    K = int(cnt64.sum())
+    assert K > 1000
+    ref = calls.render_rows(grid, model.handle, og, dg, opts, count, off, K)
+    rows = calls.render_rows_segments(grid, model.handle, opts, count, off, K, seg)
+    assert torch.equal(rows, ref)
+    assert torch.equal(calls.render_rows_segments(grid, model.handle, opts, count, off, K, seg_ns), ref)

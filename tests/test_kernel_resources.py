@@ -35,12 +35,18 @@ def test_every_source_reported(report):
 
 def test_tile_kernels_instantiated(report):
     # 3 decoder shapes x 3 precisions x 2 walks x 2 density forms x 2 feature addressings
-    # (32-bit byte offsets in render_tile.hip, 64-bit in render_tile_wide.hip)
-    assert len(_tile_kernels(report)) == 72
-    assert all(name.endswith('ELb1EEEvNS_9BatchArgsE') for name in report['render_tile.hip']
+    # (32-bit byte offsets in render_tile.hip, 64-bit in render_tile_wide.hip), plus the
+    # segment-recording training forward: 3 decoder shapes x FP32 x offline walk x channel-0
+    # density x 2 feature addressings.  Mangled names end in ...ELb<B32>ELb<SEG>EEEv.
+    ks = _tile_kernels(report)
+    assert len(ks) == 78
+    seg = [name for name in ks if name.endswith('ELb1EEEvNS_9BatchArgsE')]
+    assert len(seg) == 6
+    assert all('EEELi0ELb0ELb1ELb' in name for name in seg)  # FP32, offline walk, channel-0 density
+    assert all(('ELb1ELb0EEEv' in name or 'ELb1ELb1EEEv' in name) for name in report['render_tile.hip']
                if 'render_tile_kernel' in name)
-    assert all(name.endswith('ELb0EEEvNS_9BatchArgsE') for name in report['render_tile_wide.hip']
-               if 'render_tile_kernel' in name)
+    assert all(('ELb0ELb0EEEv' in name or 'ELb0ELb1EEEv' in name)
+               for name in report['render_tile_wide.hip'] if 'render_tile_kernel' in name)
 
 
 def test_matrix_core_tile_kernels_do_not_spill(report):
@@ -65,7 +71,7 @@ def test_matrix_core_tile_kernels_do_not_spill(report):
 def test_benchmark_kernel_fits_four_waves_per_simd(report):
     # BF16, real-time walk, density from channel 0, 32-bit feature offsets: 16 waves per CU need
     # at most 128 registers per lane (and the LDS of 16 waves, 7.75 KB each)
-    name = 'render_tile_kernelINS_3DecILi32ELi32ELi0ELb0ELi1ELi1EEELi1ELb1ELb1ELb1EE'
+    name = 'render_tile_kernelINS_3DecILi32ELi32ELi0ELb0ELi1ELi1EEELi1ELb1ELb1ELb1ELb0EE'
     (r,) = [v for k, v in _tile_kernels(report).items() if name in k]
     assert r['vgpr'] + r['agpr'] <= 128 and r['vgpr_spill'] == 0, r
     assert r['lds_bytes'] == 0  # dynamic LDS only (the wave's queue, accumulators, view terms)
