@@ -381,11 +381,21 @@ struct WgradLayout {
 
 constexpr int kWgradTK = 32;
 
+// The weight gradients' first pass.  One LDS stage (TK rows) is staged in registers: every global
+// load of the stage is issued before any is used (fully unrolled loops of compile-time trip count),
+// so a stage costs one memory round trip, and the next stage's loads are in flight while the
+// current one is summed.  The H-wide row arrays (dr1, dr2, r1, r2) move as float4 chunks; F (rows
+// 9 + C floats apart), do, ds and the directions as single floats.
 template <int C, int H>
 __global__ void __launch_bounds__((WgradLayout<C, H>::THREADS))
 wgrad_partial_kernel(WgradArgs p) {
     using L = WgradLayout<C, H>;
     constexpr int TK = kWgradTK;
+    constexpr int QH = H / 4;
+    constexpr int NQ = (TK * QH + L::THREADS - 1) / L::THREADS;
+    constexpr int NF = (TK * C + L::THREADS - 1) / L::THREADS;
+    constexpr int NS = (TK * 4 + L::THREADS - 1) / L::THREADS;
+    constexpr int ND = (TK * 3 + L::THREADS - 1) / L::THREADS;
     __shared__ __align__(16) float sa[TK][L::LA];
     __shared__ __align__(16) float sb[TK][L::LB];
     const int tid = threadIdx.x;
@@ -404,50 +414,94 @@ wgrad_partial_kernel(WgradArgs p) {
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 8; ++j) acc[i][j] = 0.0f;
-    const int stride = 9 + C;
-    for (int64_t k0 = k_begin; k0 < k_end; k0 += TK) {
-        // rows past the slab's end are staged as zeros (a zero a with a finite b adds nothing)
-        for (int i = tid; i < TK * L::LA; i += L::THREADS) {
-            const int r = i / L::LA, c = i % L::LA;
-            const int64_t k = k0 + r;
-            float v = 0.0f;
-            if (k < k_end) {
-                if (c < H) v = p.dr1[k * H + c];
-                else if (c < 2 * H) v = p.dr2[k * H + (c - H)];
-                else if (c < 2 * H + 3) v = p.dout[k * 3 + (c - 2 * H)];
-                else v = p.ds[k];
+
+    float4 q1[NQ], q2[NQ], q3[NQ], q4[NQ];  // dr1, dr2, r1, r2
+    float vf[NF], vs[NS], vd[ND];           // F, [do | ds], d
+    // rows past the slab's end are staged as zeros (a zero a with a finite b adds nothing)
+    auto load = [&](int64_t k0) {
+#pragma unroll
+        for (int u = 0; u < NQ; ++u) {
+            const int i = tid + u * L::THREADS;
+            const int64_t k = k0 + i / QH;
+            const int j = i % QH;
+            // values, not a select between a global address and a local zero (which would put
+            // the zero in scratch memory and load through the selected address)
+            q1[u] = q2[u] = q3[u] = q4[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (i < TK * QH && k < k_end) {
+                q1[u] = reinterpret_cast<const float4*>(p.dr1 + k * H)[j];
+                q2[u] = reinterpret_cast<const float4*>(p.dr2 + k * H)[j];
+                q3[u] = reinterpret_cast<const float4*>(p.r1 + k * H)[j];
+                q4[u] = reinterpret_cast<const float4*>(p.r2 + k * H)[j];
             }
-            sa[r][c] = v;
         }
-        for (int i = tid; i < TK * (C + 2 * H); i += L::THREADS) {
-            const int r = i / (C + 2 * H), c = i % (C + 2 * H);
-            const int64_t k = k0 + r;
-            float v = 0.0f;
-            int col;
-            if (c < C) {
-                col = c;
-                if (k < k_end) v = p.rows[k * stride + 9 + c];
-            } else if (c < C + H) {
-                col = L::B_R1 + (c - C);
-                if (k < k_end) v = p.r1[k * H + (c - C)];
-            } else {
-                col = L::B_R2 + (c - C - H);
-                if (k < k_end) v = p.r2[k * H + (c - C - H)];
+#pragma unroll
+        for (int u = 0; u < NF; ++u) {
+            const int i = tid + u * L::THREADS;
+            const int64_t k = k0 + i / C;
+            vf[u] = (i < TK * C && k < k_end) ? p.rows[k * (9 + C) + 9 + i % C] : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < NS; ++u) {
+            const int i = tid + u * L::THREADS;
+            const int64_t k = k0 + i / 4;
+            const int c = i % 4;
+            vs[u] = (i < TK * 4 && k < k_end) ? (c < 3 ? p.dout[k * 3 + c] : p.ds[k]) : 0.0f;
+        }
+        // the row's direction (a dependent load through ray[k])
+#pragma unroll
+        for (int u = 0; u < ND; ++u) {
+            const int i = tid + u * L::THREADS;
+            const int64_t k = k0 + i / 3;
+            vd[u] = (i < TK * 3 && k < k_end) ? p.d[p.ray[k] * 3 + i % 3] : 0.0f;
+        }
+    };
+    auto store = [&]() {
+#pragma unroll
+        for (int u = 0; u < NQ; ++u) {
+            const int i = tid + u * L::THREADS;
+            if (i < TK * QH) {
+                const int r = i / QH, c = 4 * (i % QH);
+                *reinterpret_cast<float4*>(&sa[r][c]) = q1[u];
+                *reinterpret_cast<float4*>(&sa[r][H + c]) = q2[u];
+                *reinterpret_cast<float4*>(&sb[r][L::B_R1 + c]) = q3[u];
+                *reinterpret_cast<float4*>(&sb[r][L::B_R2 + c]) = q4[u];
             }
-            sb[r][col] = v;
+        }
+#pragma unroll
+        for (int u = 0; u < NF; ++u) {
+            const int i = tid + u * L::THREADS;
+            if (i < TK * C) sb[i / C][i % C] = vf[u];
+        }
+#pragma unroll
+        for (int u = 0; u < NS; ++u) {
+            const int i = tid + u * L::THREADS;
+            if (i < TK * 4) sa[i / 4][2 * H + i % 4] = vs[u];
         }
         // PE(d) of the row's ray, element by element as positional_encoding forms it
-        for (int i = tid; i < TK * 12; i += L::THREADS) {
-            const int r = i / 12, q = i % 12, l = q / 3, ax = q % 3;
-            const int64_t k = k0 + r;
-            const float dv = k < k_end ? p.d[p.ray[k] * 3 + ax] : 0.0f;
-            float sv, cv;
-            sincospif((float)(1 << l) * dv, &sv, &cv);
-            sb[r][C + 3 + 6 * l + ax] = sv;
-            sb[r][C + 6 + 6 * l + ax] = cv;
-            if (l == 0) sb[r][C + ax] = dv;
+#pragma unroll
+        for (int u = 0; u < ND; ++u) {
+            const int i = tid + u * L::THREADS;
+            if (i < TK * 3) {
+                const int r = i / 3, ax = i % 3;
+                const float dv = vd[u];
+                sb[r][C + ax] = dv;
+#pragma unroll 1
+                for (int l = 0; l < 4; ++l) {
+                    float sv, cv;
+                    sincospif((float)(1 << l) * dv, &sv, &cv);
+                    sb[r][C + 3 + 6 * l + ax] = sv;
+                    sb[r][C + 6 + 6 * l + ax] = cv;
+                }
+            }
         }
+    };
+
+    load(k_begin);
+    for (int64_t k0 = k_begin; k0 < k_end; k0 += TK) {
+        __syncthreads();  // the previous stage's sums have read the LDS
+        store();
         __syncthreads();
+        if (k0 + TK < k_end) load(k0 + TK);
         if (active) {
 #pragma unroll 4
             for (int r = 0; r < TK; ++r) {
@@ -462,29 +516,49 @@ wgrad_partial_kernel(WgradArgs p) {
                     for (int j = 0; j < 8; ++j) acc[i][j] = fmaf(a[i], b[j], acc[i][j]);
             }
         }
-        __syncthreads();
     }
     if (active) {
         float* out = p.partial + ((int64_t)blockIdx.x * L::NT + tid) * 32;
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 8; ++j) out[i * 8 + j] = acc[i][j];
+        for (int i = 0; i < 4; ++i) {
+            *reinterpret_cast<float4*>(out + i * 8) = make_float4(acc[i][0], acc[i][1], acc[i][2], acc[i][3]);
+            *reinterpret_cast<float4*>(out + i * 8 + 4) = make_float4(acc[i][4], acc[i][5], acc[i][6], acc[i][7]);
+        }
     }
 }
+
+// Second pass: 16 gradient elements per workgroup, the workgroups' partials split over 16 slices
+// (slice s adds groups s, s + 16, ...), then the 16 slice sums added in slice order: a fixed order.
+constexpr int kWgradSlices = 16;
 
 template <int C, int H>
 __global__ void __launch_bounds__(256) wgrad_reduce_kernel(const float* __restrict__ partial,
                                                            int groups, float* __restrict__ grad) {
     using L = WgradLayout<C, H>;
-    const int e = blockIdx.x * 256 + threadIdx.x;
-    if (e >= L::NP) return;
-    int ia, ib;
-    L::element(e, ia, ib);
-    const int64_t at = (int64_t)L::tile_of(ia, ib) * 32 + (ia & 3) * 8 + (ib & 7);
-    double s = 0.0;
-    for (int g = 0; g < groups; ++g) s += (double)partial[(int64_t)g * L::NT * 32 + at];
-    grad[e] = (float)s;
+    __shared__ double red[kWgradSlices][16];
+    const int el = threadIdx.x & 15, sl = threadIdx.x >> 4;
+    const int e = blockIdx.x * 16 + el;
+    double s0 = 0.0, s1 = 0.0;
+    if (e < L::NP) {
+        int ia, ib;
+        L::element(e, ia, ib);
+        const int64_t at = (int64_t)L::tile_of(ia, ib) * 32 + (ia & 3) * 8 + (ib & 7);
+        const int64_t stride = (int64_t)L::NT * 32;
+        int g = sl;
+        for (; g + kWgradSlices < groups; g += 2 * kWgradSlices) {
+            s0 += (double)partial[g * stride + at];
+            s1 += (double)partial[(g + kWgradSlices) * stride + at];
+        }
+        if (g < groups) s0 += (double)partial[g * stride + at];
+    }
+    red[sl][el] = s0 + s1;
+    __syncthreads();
+    if (sl == 0 && e < L::NP) {
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < kWgradSlices; ++j) s += red[j][el];
+        grad[e] = (float)s;
+    }
 }
 
 int wgrad_tiles(int C, int H) {
@@ -513,7 +587,7 @@ static hipError_t launch_wgrad_ch(WgradArgs p, float* grad, hipStream_t s) {
     const int64_t groups = wgrad_groups(p.n_rows);
     p.rows_per_group = wgrad_stages_per_group(p.n_rows) * kWgradTK;
     hipLaunchKernelGGL((wgrad_partial_kernel<C, H>), dim3((unsigned)groups), dim3(L::THREADS), 0, s, p);
-    hipLaunchKernelGGL((wgrad_reduce_kernel<C, H>), dim3((L::NP + 255) / 256), dim3(256), 0, s,
+    hipLaunchKernelGGL((wgrad_reduce_kernel<C, H>), dim3((L::NP + 15) / 16), dim3(256), 0, s,
                        (const float*)p.partial, (int)groups, grad);
     return hipGetLastError();
 }

@@ -950,6 +950,9 @@ int diver_adjoint_weight_grads(const diver_decoder* dec, int64_t n_rows, const f
             (long long)need, (long long)workspace_bytes);
     REQUIRE(n_rows == 0 || (rows && ray && d && r1 && r2 && d_out && d_s && d_r1 && d_r2),
             "null pointer");
+    REQUIRE(((reinterpret_cast<uintptr_t>(r1) | reinterpret_cast<uintptr_t>(r2) |
+              reinterpret_cast<uintptr_t>(d_r1) | reinterpret_cast<uintptr_t>(d_r2)) & 15) == 0,
+            "r1, r2, d_r1 and d_r2 must be 16-byte aligned");
     diver::WgradArgs p{};
     p.n_rows = n_rows;
     p.rows = rows;
