@@ -22,12 +22,14 @@ struct AdamScalars {
     float b1, b2, omb1, omb2, eps, wd, step_size, bc2_sqrt;
 };
 
+// Every product-sum is an explicit fmaf, so no kernel's code layout can change how the compiler
+// contracts them: the dense and row kernels compute the same bits for the same element.
 __device__ __forceinline__ void adam_one(float& p, float& g, float& m, float& v, const AdamScalars& s) {
-    const float gw = s.wd != 0.0f ? g + s.wd * p : g;
-    m = s.b1 * m + s.omb1 * gw;
-    v = s.b2 * v + s.omb2 * gw * gw;
+    const float gw = s.wd != 0.0f ? fmaf(s.wd, p, g) : g;
+    m = fmaf(s.b1, m, s.omb1 * gw);
+    v = fmaf(s.b2, v, (s.omb2 * gw) * gw);
     const float denom = sqrtf(v) / s.bc2_sqrt + s.eps;
-    p = p - s.step_size * (m / denom);
+    p = fmaf(-s.step_size, m / denom, p);
 }
 
 typedef float f4 __attribute__((ext_vector_type(4)));
