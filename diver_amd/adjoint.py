@@ -70,12 +70,13 @@ class _FusedRender(Function):
         # part is pushed and the kernel keeps its channel-0 density path
         calls.decoder_update(decoder, params, first=decoder.channels + 1)
         opts = calls.render_opts('fp32', traversal, t_stop, 0.0)
-        # With a gradient to come, the offline walk records each interval's entry and exit
+        # With a gradient to come and no early termination (the reference's training), the
+        # offline walk records each interval's entry and exit
         # ([R, cap, 6], cap = the walk's step bound 3N + 8, so every interval fits), and the
         # backward builds its rows from them instead of walking every ray again.
         cap = 3 * grid.n + 8
         segments = None
-        if (any(ctx.needs_input_grad[:2]) and traversal == 'offline'
+        if (any(ctx.needs_input_grad[:2]) and traversal == 'offline' and t_stop <= 0.0
                 and getattr(grid, 'blocks', None) is not None
                 and getattr(grid, 'bricks', None) is not None
                 and o.shape[0] * cap * 24 <= SEGMENT_BYTES):

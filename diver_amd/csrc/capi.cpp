@@ -691,9 +691,9 @@ int diver_render_rays_segments(const diver_grid* grid, const diver_decoder* dec,
     REQUIRE(n_rays == 0 || (o && d && rgba && count && segments), "null pointer");
     REQUIRE(seg_cap >= 1, "seg_cap must be positive");
     REQUIRE(opts->precision == DIVER_FP32 && opts->traversal == DIVER_WALK_OFFLINE &&
-                opts->alpha_skip == 0.0f,
-            "segments are recorded by the FP32 offline walk with alpha_skip == 0 (the training "
-            "forward)");
+                opts->alpha_skip == 0.0f && opts->t_stop <= 0.0f,
+            "segments are recorded by the FP32 offline walk with alpha_skip == 0 and no early "
+            "termination (t_stop <= 0: the training forward)");
     a.n_rays = n_rays;
     a.o = o;
     a.d = d;

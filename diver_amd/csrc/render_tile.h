@@ -286,11 +286,45 @@ struct TileLane {
     float tail = 0.0f;  // the walk ends once p.d passes this (tail_limit)
 };
 
+// Phase B's density step for rows queued without it (SEG).  Lane r holds row r's channel-0 integral
+// s (the same corners, order and roundings as the walk's channel-0 sum); every row's sigma, alpha and
+// sparsity term are formed in parallel into its (already consumed) entry slots, then each owner lane
+// takes its rows in queue order, which is its walk order, and does what the walk does per interval:
+// weight = T_before * alpha, the sparsity sum, the compensated sum of sigma and T = exp(-S).  The
+// results are the same bits as the walk's.
+template <class D>
+__device__ __forceinline__ void defer_density(TileLane& L, const BatchArgs& p, float* queue, float s,
+                                              bool mine, int nbatch, int lane) {
+    if (mine) {
+        float sigma, alpha;
+        density_alpha<D>(s, sigma, alpha);
+        float* rw = queue + lane * kRowFloats;
+        rw[0] = sigma;
+        rw[1] = alpha;
+        rw[2] = p.sparsity ? sparsity_term(s) : 0.0f;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    for (int r = 0; r < nbatch; ++r) {
+        float* rw = queue + r * kRowFloats;
+        if (__builtin_bit_cast(int, rw[7]) == lane) {
+            rw[6] = L.T * rw[1];
+            if (p.sparsity) L.reg += rw[2];
+            L.S.add(rw[0]);
+            L.T = __expf(-L.S.s);
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 // Phase A: every live lane walks until the queue holds kFill rows or every lane is done.  AF: every
 // live ray of the wave takes the reciprocal-division step (the usual case); such a walk needs no
 // step bound, since each step moves one axis to its next integer plane ahead and the walk leaves
 // the grid after at most 3(N+1) steps.
-// SEG: also record each evaluated interval's entry and exit at seg_base + its index along the ray.
+// SEG (the training forward: t_stop = 0, alpha_skip = 0, channel-0 density): record each evaluated
+// interval's entry and exit at seg_base + its index along the ray, and queue it without its density
+// (no early termination, so nothing on the walk depends on it); phase B adds the densities.
 template <class D, bool RT, bool DCH0, bool AF, bool B32, bool SEG>
 __device__ __forceinline__ void phase_a(TileLane& L, const BatchArgs& p, const uint32_t* blocks,
                                         float* queue, int lane, int64_t seg_base) {
@@ -324,7 +358,20 @@ __device__ __forceinline__ void phase_a(TileLane& L, const BatchArgs& p, const u
                     ray.voxel(nx, ny, nz, vx, vy, vz);
                     px = ray.lx; py = ray.ly; pz = ray.lz;
                     ray.lx = nx; ray.ly = ny; ray.lz = nz;
-                    if (L.occ.lookup(a.mask, blocks, a.bricks, n, vx, vy, vz)) {
+                    if (SEG && L.occ.lookup(a.mask, blocks, a.bricks, n, vx, vy, vz)) {
+                        // training forward (no early termination): the row is queued without its
+                        // density, which phase B works out for the whole batch at once
+                        qx = nx; qy = ny; qz = nz;
+                        L.used += 1;
+                        if (L.used <= p.seg_cap) {
+                            float2* sp = reinterpret_cast<float2*>(p.segments + (seg_base + L.used - 1) * 6);
+                            sp[0] = make_float2(px, py);
+                            sp[1] = make_float2(pz, qx);
+                            sp[2] = make_float2(qy, qz);
+                        }
+                        wgt = 0.0f;
+                        push = true;
+                    } else if (!SEG && L.occ.lookup(a.mask, blocks, a.bricks, n, vx, vy, vz)) {
                         qx = nx; qy = ny; qz = nz;
                         float w[8];
                         uint32_t off[8];
@@ -345,15 +392,6 @@ __device__ __forceinline__ void phase_a(TileLane& L, const BatchArgs& p, const u
                         density_alpha<D>(s, sigma, alpha);
                         L.used += 1;
                         if (p.sparsity) L.reg += sparsity_term(s);
-                        if constexpr (SEG) {
-                            if (L.used <= p.seg_cap) {
-                                float2* sp = reinterpret_cast<float2*>(p.segments +
-                                                                       (seg_base + L.used - 1) * 6);
-                                sp[0] = make_float2(px, py);
-                                sp[1] = make_float2(pz, qx);
-                                sp[2] = make_float2(qy, qz);
-                            }
-                        }
                         if (alpha >= a.alpha_skip) {
                             wgt = L.T * alpha;
                             push = true;
@@ -493,6 +531,7 @@ __global__ void __launch_bounds__(64 * kWavesPerGroup, (tile_waves_per_simd<D, P
 #pragma unroll
                 for (int i = 0; i < Z; ++i) z[i] = 0.0f;
             }
+            if constexpr (SEG) defer_density<D>(L, p, queue, z[0], mine, nbatch, lane);
             float b0[Z / 2], b1[Z / 2];
             rows_to_b_f32<Z>(z, b0, b1);
             shade_tile_f32<D>(a, queue, hvs, acc, 0, nbatch, b0, lane);

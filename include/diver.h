@@ -302,8 +302,11 @@ int diver_render_rows(const diver_grid* grid, const diver_decoder* dec,
 /* The training forward with its intervals recorded: diver_render_rays_sparsity (sparsity may be
  * NULL here) that also writes, for the k-th interval ray r evaluates (k < seg_cap), its entry and
  * exit in grid units to segments [n_rays][seg_cap][6] (p_in, p_out).  FP32 offline walk,
- * alpha_skip == 0, block and brick bits, a decoder without pre-layers whose density is channel 0
- * (w_s = e_0, b_s = 0).  A ray evaluates at most 3 (N + 1) intervals, so seg_cap = 3 (N + 1)
+ * alpha_skip == 0, no early termination (t_stop <= 0, as the reference trains), block and brick
+ * bits, a decoder without pre-layers whose density is channel 0 (w_s = e_0, b_s = 0).  Without
+ * early termination the walk does not need each interval's density, so this kernel queues the
+ * intervals and evaluates their densities in the shading batches; rgba, count and sparsity are
+ * the same bits as diver_render_rays_sparsity's.  A ray evaluates at most 3 (N + 1) intervals, so seg_cap = 3 (N + 1)
  * records every one.  With diver_render_rows_segments this replaces the re-walk of
  * diver_render_rows, which recomputes the intervals the reference's autograd keeps from
  * masked_intersect (model/diver.py:87-93). */

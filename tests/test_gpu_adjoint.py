@@ -249,15 +249,15 @@ def test_render_rows_match_byte_mask_path(dev, traversal, t_stop):
     assert float((t_after - t_fwd).abs().max()) <= 1e-5
 
 
-@pytest.mark.parametrize('t_stop,hidden', [(0.0, 64), (1e-4, 64), (0.0, 32)])
-def test_recorded_segments_match_rewalk(dev, t_stop, hidden):
-    """The training forward that records each interval's entry and exit
-    (diver_render_rays_segments) returns the same rgba, counts and sparsity sums as the plain
-    forward, bit for bit, and the rows built from its segments (diver_render_rows_segments) equal
-    the re-walk's rows (diver_render_rows) bit for bit; the recorded segments past each ray's count
-    are never read."""
+@pytest.mark.parametrize('t_stop,hidden,seed', [(0.0, 64, 9), (0.0, 32, 9), (0.0, 64, 10)])
+def test_recorded_segments_match_rewalk(dev, t_stop, hidden, seed):
+    """The training forward that records each interval's entry and exit and evaluates densities in
+    the shading batches (diver_render_rays_segments) returns the same rgba, counts and sparsity sums
+    as the plain forward, which evaluates each on the walk, bit for bit; and the rows built from its
+    segments (diver_render_rows_segments) equal the re-walk's rows (diver_render_rows) bit for bit.
+    With early termination (t_stop > 0) the call is refused."""
     from diver_amd import calls
-    sc, _gt, o, d, layers = _setup(dev, n_rays=4099, hidden=hidden, seed=9)
+    sc, _gt, o, d, layers = _setup(dev, n_rays=4099, hidden=hidden, seed=seed)
     grid = SceneGrid.from_synthetic(sc, dev)
     model = FusedRenderer(grid, BenchmarkDecoder(layers, 32).to(dev), traversal='offline')
     og, dg = torch.from_numpy(o).to(dev), torch.from_numpy(d).to(dev)
@@ -279,3 +279,7 @@ def test_recorded_segments_match_rewalk(dev, t_stop, hidden):
     rows = calls.render_rows_segments(grid, model.handle, opts, count, off, K, seg)
     assert torch.equal(rows, ref)
     assert torch.equal(calls.render_rows_segments(grid, model.handle, opts, count, off, K, seg_ns), ref)
+    from diver_amd._lib import DiverError
+    with pytest.raises(DiverError):
+        calls.render_rays_segments(grid, model.handle, og, dg,
+                                   calls.render_opts('fp32', 'offline', 1e-4, 0.0), cap)
