@@ -112,7 +112,7 @@ hipError_t launch_alpha_map(int variant, const RenderArgs& a, int pad, float* ma
                             const float* pack_pre, hipStream_t s);
 // Decoder and compositing adjoint over the recompute pass's rows (adjoint_mlp.hip).
 struct AdjArgs {
-    int64_t n_rows;          // rows (a bucket; rows past n_valid are zero)
+    int64_t n_rows;          // rows (any rows past n_valid are zero padding)
     int64_t n_valid;         // composited intervals
     const float* rows;       // [n_rows][row_stride]: p_in, p_out, s, T_before, alpha, F
     int row_stride;          // 9 + C
