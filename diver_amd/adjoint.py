@@ -112,13 +112,15 @@ class _FusedRender(Function):
         C, H = decoder.channels, decoder.hidden
         R = o.shape[0]
         cnt64 = count.to(torch.int64)
-        row_off = torch.cumsum(cnt64, 0) - cnt64
-        K = int(cnt64.sum().item())
+        ends = torch.cumsum(cnt64, 0)
+        row_off = ends - cnt64
+        # queued ahead of the one host sync (the row count): the decoder's packed operands
+        calls.decoder_update(decoder, params, first=decoder.channels + 1)
+        K = int(ends[-1].item())
         dev = o.device
         if K == 0:
             gv0 = None if ctx.accumulate is not None else torch.zeros(ctx.vshape, device=dev)
             return gv0, torch.zeros_like(params), *([None] * 9)
-        calls.decoder_update(decoder, params, first=decoder.channels + 1)
         if segments is not None:
             rows = calls.render_rows_segments(grid, decoder, ctx.opts, count, row_off, K, segments)
         else:
