@@ -146,7 +146,7 @@ def _declare(L):
         'diver_render_rays_segments': [ctypes.POINTER(Grid), vp, ctypes.POINTER(RenderOpts), i64,
                                        vp, vp, vp, vp, vp, vp, i32, vp],
         'diver_render_rows_segments': [ctypes.POINTER(Grid), vp, ctypes.POINTER(RenderOpts), i64,
-                                       vp, vp, vp, i32, vp, vp],
+                                       vp, vp, vp, i32, vp, vp, vp],
     }
     sig.update({
         'diver_server_create': [ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)],

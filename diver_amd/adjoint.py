@@ -122,10 +122,11 @@ class _FusedRender(Function):
             gv0 = None if ctx.accumulate is not None else torch.zeros(ctx.vshape, device=dev)
             return gv0, torch.zeros_like(params), *([None] * 9)
         if segments is not None:
-            rows = calls.render_rows_segments(grid, decoder, ctx.opts, count, row_off, K, segments)
+            rows, ray = calls.render_rows_segments(grid, decoder, ctx.opts, count, row_off, K,
+                                                   segments, with_ray=True)
         else:
             rows = calls.render_rows(grid, decoder, o, d, ctx.opts, count, row_off, K)
-        ray = torch.repeat_interleave(torch.arange(R, device=dev), cnt64, output_size=K)
+            ray = torch.repeat_interleave(torch.arange(R, device=dev), cnt64, output_size=K)
         pin, pout = rows[:, 0:3], rows[:, 3:6]
         # the decoder over the rows on the matrix cores: relu(y1), relu(y2), colour, weight
         r1, r2, c, w = calls.adjoint_decoder_forward(decoder, rows, ray, d)

@@ -443,9 +443,10 @@ def render_rays_segments(grid, decoder, o, d, opts, seg_cap, sparsity=True):
     return rgba, count, sreg, segments
 
 
-def render_rows_segments(grid, decoder, opts, count, row_offset, n_rows, segments):
+def render_rows_segments(grid, decoder, opts, count, row_offset, n_rows, segments, with_ray=False):
     """diver_render_rows from recorded segments [R, seg_cap, 6] (every count <= seg_cap):
-    rows [n_rows, 9 + C], the same as render_rows' re-walk."""
+    rows [n_rows, 9 + C], the same as render_rows' re-walk; with_ray: (rows, ray [n_rows] int64,
+    the ray of each row)."""
     dev = _dev(grid.voxels, grid.mask, count, row_offset, segments)
     count = _i32(count, 'count')
     if row_offset.dtype != torch.int64:
@@ -456,11 +457,12 @@ def render_rows_segments(grid, decoder, opts, count, row_offset, n_rows, segment
     if count.shape[0] != R:
         raise ValueError('count and segments disagree on the ray count')
     rows = torch.empty((int(n_rows), 9 + grid.channels), dtype=torch.float32, device=dev)
+    ray = torch.empty((int(n_rows),), dtype=torch.int64, device=dev) if with_ray else None
     g = _grid_of(grid)
     _lib.call('diver_render_rows_segments', ctypes.byref(g), decoder.handle, ctypes.byref(opts), R,
               count.data_ptr(), row_offset.data_ptr(), segments.data_ptr(), int(cap),
-              rows.data_ptr(), _s(count))
-    return rows
+              rows.data_ptr(), _lib.ptr(ray), _s(count))
+    return (rows, ray) if with_ray else rows
 
 
 def corner_keys(n1, coord_in, coord_out):

@@ -302,7 +302,8 @@ template <class D>
 __global__ void __launch_bounds__(256) rows_segments_kernel(int64_t R, const int32_t* __restrict__ count,
                                                             const int64_t* __restrict__ row_off,
                                                             const float* __restrict__ seg, int cap,
-                                                            float* __restrict__ rows) {
+                                                            float* __restrict__ rows,
+                                                            int64_t* __restrict__ ray) {
     constexpr int RW = 9 + D::C;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t r = t >> 3;
@@ -319,15 +320,16 @@ __global__ void __launch_bounds__(256) rows_segments_kernel(int64_t R, const int
         o[3] = a1.y;
         o[4] = a2.x;
         o[5] = a2.y;
+        if (ray) ray[row_off[r] + k] = r;
     }
 }
 
 template <class D>
 static hipError_t launch_rows_segments_d(const RenderArgs& a, bool dch0, const int32_t* count,
                                          const int64_t* off, const float* seg, int cap, float* rows,
-                                         hipStream_t s) {
+                                         int64_t* ray, hipStream_t s) {
     hipLaunchKernelGGL((rows_segments_kernel<D>), dim3((unsigned)((a.n_rays * 8 + 255) / 256)), dim3(256),
-                       0, s, a.n_rays, count, off, seg, cap, rows);
+                       0, s, a.n_rays, count, off, seg, cap, rows, ray);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return launch_rows_fill<D>(a, dch0, count, off, rows, s);
@@ -335,12 +337,12 @@ static hipError_t launch_rows_segments_d(const RenderArgs& a, bool dch0, const i
 
 hipError_t launch_rows_from_segments(int variant, const RenderArgs& a, bool dch0, const int32_t* count,
                                      const int64_t* off, const float* segments, int seg_cap,
-                                     float* rows, hipStream_t s) {
+                                     float* rows, int64_t* ray, hipStream_t s) {
     if (a.n_rays == 0) return hipSuccess;
     switch (variant) {
-        case 0: return launch_rows_segments_d<DecB32>(a, dch0, count, off, segments, seg_cap, rows, s);
-        case 1: return launch_rows_segments_d<DecB64>(a, dch0, count, off, segments, seg_cap, rows, s);
-        case 2: return launch_rows_segments_d<DecB64C64>(a, dch0, count, off, segments, seg_cap, rows, s);
+        case 0: return launch_rows_segments_d<DecB32>(a, dch0, count, off, segments, seg_cap, rows, ray, s);
+        case 1: return launch_rows_segments_d<DecB64>(a, dch0, count, off, segments, seg_cap, rows, ray, s);
+        case 2: return launch_rows_segments_d<DecB64C64>(a, dch0, count, off, segments, seg_cap, rows, ray, s);
         default: return hipErrorInvalidValue;
     }
 }

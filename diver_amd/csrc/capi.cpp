@@ -841,7 +841,7 @@ int diver_render_rows(const diver_grid* grid, const diver_decoder* dec,
 int diver_render_rows_segments(const diver_grid* grid, const diver_decoder* dec,
                                const diver_render_opts* opts, int64_t n_rays, const int32_t* count,
                                const int64_t* row_offset, const float* segments, int seg_cap,
-                               float* rows, diver_stream_t stream) {
+                               float* rows, int64_t* ray, diver_stream_t stream) {
     diver::RenderArgs a{};
     int st = render_common(grid, dec, opts, a);
     if (st != DIVER_OK) return st;
@@ -853,7 +853,7 @@ int diver_render_rows_segments(const diver_grid* grid, const diver_decoder* dec,
     REQUIRE(n_rays == 0 || (count && row_offset && segments && rows), "null pointer");
     a.n_rays = n_rays;
     return hip_status(diver::launch_rows_from_segments(dec->variant, a, dec->density_ch0, count,
-                                                       row_offset, segments, seg_cap, rows,
+                                                       row_offset, segments, seg_cap, rows, ray,
                                                        S(stream)),
                       "render_rows_segments");
 }

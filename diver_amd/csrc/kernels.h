@@ -196,7 +196,7 @@ hipError_t launch_ray_suffix(int64_t R, const int64_t* off, const int32_t* cnt, 
 // The rows from the forward's recorded segments instead of a re-walk (the same rows, bit for bit).
 hipError_t launch_rows_from_segments(int variant, const RenderArgs& a, bool dch0, const int32_t* count,
                                      const int64_t* off, const float* segments, int seg_cap,
-                                     float* rows, hipStream_t s);
+                                     float* rows, int64_t* ray, hipStream_t s);
 hipError_t launch_rows(int variant, const RenderArgs& a, bool dch0, const int32_t* count,
                        const int64_t* off, float* rows, hipStream_t s);
 hipError_t launch_camera_rays(const CameraDesc& cam, float* o, float* d, hipStream_t s);

@@ -316,11 +316,12 @@ int diver_render_rays_segments(const diver_grid* grid, const diver_decoder* dec,
                                float* segments, int seg_cap, diver_stream_t stream);
 
 /* diver_render_rows from the segments of diver_render_rays_segments instead of a re-walk: the same
- * rows bit for bit, provided every count <= seg_cap (rows past seg_cap of a ray are not written). */
+ * rows bit for bit, provided every count <= seg_cap (rows past seg_cap of a ray are not written).
+ * ray [sum(count)] (int64, may be NULL) receives the ray of each row. */
 int diver_render_rows_segments(const diver_grid* grid, const diver_decoder* dec,
                                const diver_render_opts* opts, int64_t n_rays, const int32_t* count,
                                const int64_t* row_offset, const float* segments, int seg_cap,
-                               float* rows, diver_stream_t stream);
+                               float* rows, int64_t* ray, diver_stream_t stream);
 
 /* Per-ray suffix sums for the compositing adjoint over the rows of diver_render_rows: for row k of
  * ray r (rows row_offset[r] .. row_offset[r] + count[r] - 1),

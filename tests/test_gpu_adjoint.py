@@ -276,8 +276,9 @@ def test_recorded_segments_match_rewalk(dev, t_stop, hidden, seed):
     K = int(cnt64.sum())
     assert K > 1000
     ref = calls.render_rows(grid, model.handle, og, dg, opts, count, off, K)
-    rows = calls.render_rows_segments(grid, model.handle, opts, count, off, K, seg)
+    rows, ray = calls.render_rows_segments(grid, model.handle, opts, count, off, K, seg, with_ray=True)
     assert torch.equal(rows, ref)
+    assert torch.equal(ray, torch.repeat_interleave(torch.arange(count.shape[0], device=dev), cnt64))
     assert torch.equal(calls.render_rows_segments(grid, model.handle, opts, count, off, K, seg_ns), ref)
     from diver_amd._lib import DiverError
     with pytest.raises(DiverError):
